@@ -1,0 +1,191 @@
+"""Benchmark of the Black-Scholes map on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+
+A step is one pass of the price map over the 10,000,000-option FP32 book
+(SoA: s, k, r, v, t float32 + uint8 type in, float32 price out; 25 bytes per
+option), one kernel launch per pass, as the reference's NUM_RUNS loop
+(blackscholes.c:316-347).  Inputs are resident in HBM before timing starts.
+Passes rotate over --books device copies of the book (4 x 250 MB), so the
+256 MB MALL cannot hold the working set between passes.
+
+Prints one JSON line: options/s over the timed passes, the roofline of the
+price kernel (algorithmic bytes / average launch time vs 8 TB/s), and the
+CPU baseline (the oracle's FastFlow-style static-block port on the host's
+cores, bounded sample).  Extra sections: the fused Greeks kernel's time ratio
+to price-only and the FP64 map.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from p3arsec_amd import books, capi  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("options/sec & % HBM roofline (10M fp32); Greeks cost ratio; serve p50/p99 ms")
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_passes(launch, nbooks: int, steps: int, warmup: int, stream) -> float:
+    """Runs warmup + steps launches (rotating books); returns ms per step of the
+    timed steps, from HIP events on the launch stream."""
+    for i in range(warmup):
+        launch(i % nbooks)
+    capi.call("bs_stream_sync", stream)
+    e0, e1 = capi.Event(), capi.Event()
+    capi.call("bs_device_sync")
+    e0.record(stream)
+    for i in range(steps):
+        launch(i % nbooks)
+    e1.record(stream)
+    e1.sync()
+    capi.call("bs_device_sync")
+    ms = e0.elapsed_ms(e1) / steps
+    e0.destroy()
+    e1.destroy()
+    return ms
+
+
+def cpu_baseline(book_host, passes: int) -> dict:
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    threads = max(1, min(aff, 16))
+    _, sec, used = pyoracle.price_f32(book_host, nthreads=threads, passes=passes)
+    rate = book_host.n * passes / sec
+    return {"value": rate, "unit": "options/s", "cores": used, "kind": "port",
+            "sample": f"{book_host.n} options x {passes} passes, static blocks, "
+                      f"{used} pinned threads, oracle/bs_oracle.c (bit-exact with "
+                      f"blackscholes.c), {sec:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--books", type=int, default=4)
+    ap.add_argument("--cpu-passes", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the map needs no collectives; sharding is described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    capi.load()
+    if capi.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        return 1
+    capi.call("bs_set_device", 0)
+    n, nb = args.n, args.books
+    log(f"bench: generating {n} options")
+    host = books.make_book(n, books.SEED_BASE + 2)
+    dbooks = [capi.DeviceBook(host) for _ in range(nb)]
+    outs = [capi.alloc_out(n) for _ in range(nb)]
+    stream = capi.Stream()
+    L = capi.load()
+
+    def launch_price(i):
+        capi.check(L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle), "price")
+
+    log("bench: price map")
+    ms = timed_passes(launch_price, nb, args.steps, args.warmup, stream.handle)
+    value = n / (ms * 1e-3)
+    alg_bytes = 25 * n
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+
+    # correctness of what was timed (sampled against the oracle)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    got = outs[0].download(np.float32, n)
+    idx = np.random.default_rng(0).choice(n, 200_000, replace=False)
+    sub = host.slice(0, n)
+    sub = books.Book(*(a[idx] for a in sub.arrays()))
+    ref, _, _ = pyoracle.price_f32(sub)
+    max_err = float(np.max(np.abs(got[idx].astype(np.float64) - ref)))
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "options/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "Black-Scholes price map, 10M-option FP32 book, SoA "
+                               "(5 x f32 + u8 in, f32 out), one launch per pass",
+                   "options": n, "rotating_books": nb, "bytes_per_option": 25,
+                   "max_abs_err_vs_oracle_sample": max_err},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+
+    if not args.no_extra:
+        def launch_fast(i):
+            capi.check(L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle),
+                       "fast")
+        log("bench: fast price map")
+        fast_ms = timed_passes(launch_fast, nb, args.steps, args.warmup, stream.handle)
+        result["fast_variant"] = {"ms_per_step": fast_ms, "options_per_s": n / (fast_ms * 1e-3),
+                                  "achieved_gbs": alg_bytes / (fast_ms * 1e-3) / 1e9}
+        gouts = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
+
+        def launch_greeks(i):
+            capi.check(L.bs_price_greeks_f32(*dbooks[i].ptrs(), *[o.ptr for o in gouts[i]],
+                                             n, stream.handle), "greeks")
+        log("bench: greeks map")
+        gms = timed_passes(launch_greeks, nb, max(10, args.steps // 2), 5, stream.handle)
+        result["greeks"] = {"ms_per_step": gms, "ratio_to_price": gms / ms,
+                            "options_per_s": n / (gms * 1e-3),
+                            "achieved_gbs": 41 * n / (gms * 1e-3) / 1e9}
+        for row in gouts:
+            for o in row:
+                o.free()
+        # FP64 map: one book copy in double (2 rotating copies = 980 MB)
+        log("bench: fp64 map")
+        h64 = host.astype(np.float64)
+        d64 = [capi.DeviceBook(h64, dtype=np.float64) for _ in range(2)]
+        o64 = [capi.alloc_out(n, np.float64) for _ in range(2)]
+
+        def launch_f64(i):
+            capi.check(L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, stream.handle), "f64")
+        fms = timed_passes(launch_f64, 2, max(10, args.steps // 4), 3, stream.handle)
+        result["fp64"] = {"ms_per_step": fms, "ratio_to_fp32": fms / ms,
+                          "options_per_s": n / (fms * 1e-3),
+                          "achieved_gbs": 49 * n / (fms * 1e-3) / 1e9}
+        for b in d64:
+            b.free()
+        for o in o64:
+            o.free()
+
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(host, args.cpu_passes)
+
+    print(json.dumps(result), flush=True)
+    for b in dbooks:
+        b.free()
+    for o in outs:
+        o.free()
+    stream.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

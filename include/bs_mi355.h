@@ -1,0 +1,233 @@
+/*
+ * bs_mi355.h -- C ABI of libbs_mi355.so, the MI355X (gfx950) Black-Scholes map.
+ *
+ * Reference: ParaGroup/p3arsec, PARSEC blackscholes
+ *   (parsec-ff/pkgs/apps/blackscholes/src/blackscholes.c, "BS" below).
+ * The reference compiles its pricing path into one C++ translation unit with
+ * no exported header; each entry point below names the interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns int status: BS_OK (0) or a negative BS_ERR_*.
+ *     bs_last_error() gives a message for the calling thread.  No C++
+ *     exception crosses this boundary.
+ *   - Book arrays are structure-of-arrays: spot s, strike k, rate r,
+ *     volatility v, time t (years), option type (0 = call, nonzero = put;
+ *     the reference maps 'P' -> 1, everything else -> 0, BS:761).
+ *   - Functions taking `bs_stream_t` expect DEVICE pointers (hipMalloc'd or
+ *     pinned host memory) and run asynchronously on that HIP stream (NULL =
+ *     the null stream).  Functions named *_host take host pointers and are
+ *     synchronous.  Pointers are validated (hipPointerGetAttributes) unless
+ *     bs_config_set(BS_CFG_CHECK_POINTERS, 0).
+ *   - 16-byte aligned arrays (4-byte for uint8 types) take the vector path;
+ *     anything else the scalar path.  Results are identical either way.
+ *   - Arithmetic: FP32 entry points keep the reference's float rounding step
+ *     by step (correctly rounded division and sqrt, expf/logf evaluated in
+ *     FP64 and rounded once, the double literals applied with one rounding);
+ *     *_fast_* uses hardware transcendentals instead.  FP64 entry points are
+ *     the fptype=double build of the same algorithm.
+ */
+#ifndef BS_MI355_H
+#define BS_MI355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *bs_stream_t; /* a hipStream_t */
+
+enum {
+    BS_OK = 0,
+    BS_ERR_INVALID = -1,   /* bad argument (null pointer, n < 0, ...)        */
+    BS_ERR_HIP = -2,       /* a HIP runtime call failed                      */
+    BS_ERR_NOMEM = -3,     /* device allocation failed                       */
+    BS_ERR_BUDGET = -4,    /* request larger than the context's HBM budget   */
+    BS_ERR_NOT_FOUND = -5, /* unknown book or ticket                         */
+    BS_ERR_CANCELLED = -6, /* ticket was cancelled                           */
+    BS_ERR_TIMEOUT = -7,   /* bs_wait timed out                              */
+    BS_ERR_BUSY = -8,      /* book still referenced by queued requests       */
+    BS_ERR_NODEVICE = -9,  /* no HIP device / device index out of range      */
+    BS_ERR_FAILED = -10,   /* a request's launch failed                      */
+    BS_ERR_SHUTDOWN = -11  /* context is being destroyed                     */
+};
+
+const char *bs_status_string(int status);
+const char *bs_last_error(void);
+int bs_version(void); /* 10000*major + 100*minor + patch */
+
+/* ---- configuration (process-wide) -------------------------------------- */
+enum {
+    BS_CFG_CHECK_POINTERS = 1, /* 1 (default): validate device pointers      */
+    BS_CFG_MAX_BLOCKS = 2,     /* grid cap for streaming kernels, 0 = auto    */
+    BS_CFG_BLOCK_THREADS = 3,  /* read-only: threads per block (256)          */
+    BS_CFG_UNROLL = 4,         /* 4-option groups per thread per step (1,2,4)*/
+    BS_CFG_NONTEMPORAL = 5     /* 1 (default): non-temporal loads and stores  */
+};
+int bs_config_set(int key, int64_t value);
+int bs_config_get(int key, int64_t *value);
+
+/* ---- per-option functions ------------------------------------------------
+ * fptype CNDF(fptype InputX)                                   BS:126-184
+ * fptype BlkSchlsEqEuroNoDiv(fptype sptprice, fptype strike, fptype rate,
+ *        fptype volatility, fptype time, int otype, float timet) BS:190-258
+ * Batched over device arrays (the reference's per-option calls become one
+ * map).  timet is unused by the reference and is not taken here.
+ */
+int bs_cndf_f32(const float *x, float *out, int64_t n, bs_stream_t stream);
+int bs_cndf_f64(const double *x, double *out, int64_t n, bs_stream_t stream);
+
+/* Scalar convenience forms with the reference's signatures: one device round
+ * trip per call (for API parity and spot checks, not for throughput). */
+int bs_cndf_scalar_f32(float x, float *out);
+int bs_price_scalar_f32(float sptprice, float strike, float rate, float volatility,
+                        float time, int otype, float timet, float *out);
+
+/* ---- batch pricing -------------------------------------------------------
+ * Replaces one pass of the FastFlow ParallelFor over the book
+ * (BS:315-348; ff_Map::parallel_for_thid, ff/map.hpp:120-124), the SkePU2
+ * Map<7> call (blackscholes_skepu_omp.cpp:725-728) and the SSE pointer batch
+ * BlkSchlsEqEuroNoDiv(fptype*, int, fptype*, ..., int*, float)
+ * (blackscholes.simd.c:221-223): price[i] = BlkSchlsEqEuroNoDiv(s[i], k[i],
+ * r[i], v[i], t[i], otype[i], 0) for 0 <= i < n.
+ */
+int bs_price_f32(const float *s, const float *k, const float *r, const float *v,
+                 const float *t, const uint8_t *otype, float *price, int64_t n,
+                 bs_stream_t stream);
+/* Same with the reference's int32 option-type array (BS:757-761). */
+int bs_price_i32type_f32(const float *s, const float *k, const float *r, const float *v,
+                         const float *t, const int32_t *otype, float *price, int64_t n,
+                         bs_stream_t stream);
+/* Fast variant: hardware transcendentals (v_exp_f32, v_log_f32, v_rcp_f32,
+ * v_sqrt_f32) and a Horner polynomial; about a third of the VALU work of
+ * bs_price_f32, but prices may differ from the reference by slightly more
+ * than 1e-4 (mean 6e-6). */
+int bs_price_fast_f32(const float *s, const float *k, const float *r, const float *v,
+                      const float *t, const uint8_t *otype, float *price, int64_t n,
+                      bs_stream_t stream);
+/* fptype = double build of the same map (BS:85 set to double). */
+int bs_price_f64(const double *s, const double *k, const double *r, const double *v,
+                 const double *t, const uint8_t *otype, double *price, int64_t n,
+                 bs_stream_t stream);
+
+/* Fused adjoint: price plus delta = dP/dS, vega = dP/dsigma, rho = dP/dr and
+ * theta = -dP/dT (per year) of the same FP32 price, in one pass.  No
+ * reference counterpart (the reference prices only); it extends BS:190-258. */
+int bs_price_greeks_f32(const float *s, const float *k, const float *r, const float *v,
+                        const float *t, const uint8_t *otype, float *price, float *delta,
+                        float *vega, float *rho, float *theta, int64_t n,
+                        bs_stream_t stream);
+
+/* AoS -> SoA repack on the device (BS:747-767).  `aos` points at n records
+ * laid out as the reference's OptionData (BS:89-100): float s, strike, r,
+ * divq, v, t; char OptionType; float divs, DGrefval -- 36 bytes each.  The
+ * type becomes 1 for 'P' and 0 otherwise. */
+int bs_repack_aos_f32(const void *aos, int64_t n, float *s, float *k, float *r, float *v,
+                      float *t, uint8_t *otype, bs_stream_t stream);
+
+/* Host-pointer forms (the SkePU2 CUDA back-end's updateDevice / kernel /
+ * updateHost sequence, skepu2/backend/impl/map/map_cu.inl:15-58): copy in,
+ * run `passes` passes (>= 1), copy the price out.  Synchronous. */
+int bs_price_host_f32(const float *s, const float *k, const float *r, const float *v,
+                      const float *t, const int32_t *otype, float *price, int64_t n,
+                      int passes, int device);
+
+/* bs_thread (BS:607-658) drop-in.  bs_bind_globals() records the caller's
+ * globals (BS:102-113); bs_thread_compat(&tid) then prices thread tid's block
+ * [tid*(N/nthreads), +N/nthreads) NUM_RUNS = 100 times on the GPU, or the
+ * whole book when nthreads == 1 (serial and OpenMP drivers).  Returns 0. */
+int bs_bind_globals(int num_options, int nthreads, float *sptprice, float *strike,
+                    float *rate, float *volatility, float *otime, int *otype,
+                    float *prices);
+int bs_thread_compat(void *tid_ptr);
+
+/* ---- device plumbing for hosts without a GPU runtime of their own ------- */
+int bs_device_count(int *count);
+int bs_set_device(int device);
+int bs_malloc(void **ptr, size_t bytes);
+int bs_free(void *ptr);
+int bs_memcpy_h2d(void *dst, const void *src, size_t bytes, bs_stream_t stream);
+int bs_memcpy_d2h(void *dst, const void *src, size_t bytes, bs_stream_t stream);
+int bs_memset(void *dst, int value, size_t bytes, bs_stream_t stream);
+int bs_stream_create(bs_stream_t *stream);
+int bs_stream_destroy(bs_stream_t stream);
+int bs_stream_sync(bs_stream_t stream);
+int bs_device_sync(void);
+typedef void *bs_event_t;
+int bs_event_create(bs_event_t *ev);
+int bs_event_destroy(bs_event_t ev);
+int bs_event_record(bs_event_t ev, bs_stream_t stream);
+int bs_event_sync(bs_event_t ev);
+int bs_event_elapsed_ms(bs_event_t start, bs_event_t stop, float *ms);
+
+/* ---- serving: resident books, continuous batching -------------------------
+ * No reference counterpart: the reference is a single-shot batch benchmark
+ * (BS:664-960).  A context owns one HBM arena of `hbm_budget_bytes`; books
+ * registered into it stay resident; requests against resident books are
+ * queued, admitted when their output buffers fit in the arena, fused with the
+ * other queued requests into one persistent-grid launch replayed from a
+ * captured hipGraph, and completed in arrival order of their launch.
+ * A context is thread-safe. */
+typedef struct bs_ctx_s *bs_ctx_t;
+typedef int64_t bs_book_t;
+typedef int64_t bs_ticket_t;
+
+enum { BS_REQ_PRICE = 1, BS_REQ_GREEKS = 2 };
+enum {
+    BS_TICKET_QUEUED = 0,
+    BS_TICKET_RUNNING = 1,
+    BS_TICKET_DONE = 2,
+    BS_TICKET_FAILED = 3,
+    BS_TICKET_CANCELLED = 4
+};
+
+typedef struct {
+    int64_t max_batch_requests; /* requests fused per launch (<= 4096)       */
+    int64_t max_batch_options;  /* options fused per launch                  */
+    int32_t use_graph;          /* 1: replay a captured hipGraph per launch  */
+    int32_t grid_blocks;        /* persistent grid size, 0 = auto            */
+} bs_serve_opts;
+
+typedef struct {
+    int64_t arena_bytes, arena_used, arena_peak;
+    int64_t books_resident, requests_queued, requests_running;
+    int64_t requests_done, requests_failed, requests_cancelled;
+    int64_t launches, options_priced, admission_waits;
+} bs_serve_stats;
+
+int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts *opts,
+                  bs_ctx_t *ctx);
+int bs_ctx_destroy(bs_ctx_t ctx);
+/* Uploads a book (host arrays) into the arena; BS_ERR_BUDGET when it cannot
+ * fit next to the resident books and admitted outputs. */
+int bs_book_register(bs_ctx_t ctx, const float *s, const float *k, const float *r,
+                     const float *v, const float *t, const uint8_t *otype, int64_t n,
+                     bs_book_t *book);
+/* Books still referenced by queued or running requests: BS_ERR_BUSY. */
+int bs_book_release(bs_ctx_t ctx, bs_book_t book);
+int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t *ticket);
+int bs_poll(bs_ctx_t ctx, bs_ticket_t ticket, int *state);
+/* timeout_us < 0 waits forever. */
+int bs_wait(bs_ctx_t ctx, bs_ticket_t ticket, int64_t timeout_us);
+/* Copies a finished request's outputs to host arrays (NULL skips one; the
+ * Greeks arrays only for BS_REQ_GREEKS). */
+int bs_result_copy(bs_ctx_t ctx, bs_ticket_t ticket, float *price, float *delta,
+                   float *vega, float *rho, float *theta);
+/* Device pointers of a finished request's outputs (valid until release). */
+int bs_result_device(bs_ctx_t ctx, bs_ticket_t ticket, const float **price,
+                     const float **delta, const float **vega, const float **rho,
+                     const float **theta);
+/* Frees the request's outputs in the arena and forgets the ticket. */
+int bs_result_release(bs_ctx_t ctx, bs_ticket_t ticket);
+/* Cancels a queued request (running ones finish). */
+int bs_cancel(bs_ctx_t ctx, bs_ticket_t ticket);
+/* Nanosecond timestamps (CLOCK_MONOTONIC) of submit and completion. */
+int bs_ticket_times(bs_ctx_t ctx, bs_ticket_t ticket, int64_t *submit_ns,
+                    int64_t *done_ns);
+int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BS_MI355_H */

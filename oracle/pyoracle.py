@@ -1,0 +1,137 @@
+"""ctypes front of liboracle.so -- the CPU oracle for the Black-Scholes map.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product library never uses
+it.  Every function cites the reference lines it restates in bs_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+
+
+def build() -> None:
+    """Compile liboracle.so (and the reference under _ref/ when present)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.bso_cndf_f32.argtypes = [ctypes.c_float]
+        L.bso_cndf_f32.restype = ctypes.c_float
+        L.bso_cndf_f64.argtypes = [ctypes.c_double]
+        L.bso_cndf_f64.restype = ctypes.c_double
+        L.bso_price_f32.argtypes = [ctypes.c_float] * 5 + [ctypes.c_int]
+        L.bso_price_f32.restype = ctypes.c_float
+        L.bso_price_f64.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+        L.bso_price_f64.restype = ctypes.c_double
+        L.bso_price_exact.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+        L.bso_price_exact.restype = ctypes.c_double
+        L.bso_phi_exact.argtypes = [ctypes.c_double]
+        L.bso_phi_exact.restype = ctypes.c_double
+        L.bso_cndf_f32_vec.argtypes = [_i64, _f32p, _f32p]
+        L.bso_cndf_f64_vec.argtypes = [_i64, _f64p, _f64p]
+        L.bso_price_f64_vec.argtypes = [_i64] + [_f64p] * 5 + [_u8p, _f64p]
+        L.bso_price_exact_vec.argtypes = [_i64] + [_f32p] * 5 + [_u8p, _f64p]
+        L.bso_greeks_exact_vec.argtypes = [_i64] + [_f32p] * 5 + [_u8p, _f64p]
+        L.bso_greeks_fd_vec.argtypes = [_i64] + [_f32p] * 5 + [_u8p, ctypes.c_double,
+                                                              _f64p]
+        L.bso_batch_f32.argtypes = ([_i64] + [_f32p] * 5 + [_u8p, _f32p, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)])
+        L.bso_batch_f32.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def cndf_f32(x):
+    """CNDF, fptype=float (blackscholes.c:126-184)."""
+    x = _c(x, np.float32)
+    out = np.empty_like(x)
+    lib().bso_cndf_f32_vec(x.size, x.ravel(), out.ravel())
+    return out
+
+
+def cndf_f64(x):
+    """CNDF, fptype=double."""
+    x = _c(x, np.float64)
+    out = np.empty_like(x)
+    lib().bso_cndf_f64_vec(x.size, x.ravel(), out.ravel())
+    return out
+
+
+def price_f32(book, nthreads: int = 0, passes: int = 1):
+    """BlkSchlsEqEuroNoDiv over a book (blackscholes.c:190-258) through the
+    static-block driver (blackscholes.c:315-348).  Returns (prices, seconds,
+    threads)."""
+    a = [_c(x, np.float32) for x in (book.s, book.k, book.r, book.v, book.t)]
+    ty = _c(book.otype, np.uint8)
+    out = np.empty(book.n, np.float32)
+    used = ctypes.c_int(0)
+    sec = lib().bso_batch_f32(book.n, *a, ty, out, nthreads, passes, 1,
+                              ctypes.byref(used))
+    if sec < 0:
+        raise RuntimeError("bso_batch_f32 failed")
+    return out, sec, used.value
+
+
+def price_f64(book):
+    """fptype=double restatement of BlkSchlsEqEuroNoDiv."""
+    a = [_c(x, np.float64) for x in (book.s, book.k, book.r, book.v, book.t)]
+    out = np.empty(book.n, np.float64)
+    lib().bso_price_f64_vec(book.n, *a, _c(book.otype, np.uint8), out)
+    return out
+
+
+def price_exact(book):
+    """FP64 closed form with the exact normal CDF."""
+    a = [_c(x, np.float32) for x in (book.s, book.k, book.r, book.v, book.t)]
+    out = np.empty(book.n, np.float64)
+    lib().bso_price_exact_vec(book.n, *a, _c(book.otype, np.uint8), out)
+    return out
+
+
+def greeks_exact(book):
+    """Closed-form (price, delta, vega, rho, theta), shape (5, n)."""
+    a = [_c(x, np.float32) for x in (book.s, book.k, book.r, book.v, book.t)]
+    out = np.empty((5, book.n), np.float64)
+    lib().bso_greeks_exact_vec(book.n, *a, _c(book.otype, np.uint8), out.ravel())
+    return out
+
+
+def greeks_fd(book, h: float = 1e-3):
+    """FP64 central differences of the A&S price, shape (5, n)."""
+    a = [_c(x, np.float32) for x in (book.s, book.k, book.r, book.v, book.t)]
+    out = np.empty((5, book.n), np.float64)
+    lib().bso_greeks_fd_vec(book.n, *a, _c(book.otype, np.uint8), float(h), out.ravel())
+    return out
+
+
+REF_DIR = os.path.join(_HERE, "_ref")
+
+
+def ref_binary(kind: str = "serial") -> str | None:
+    """Path of the compiled reference (oracle/_ref/blackscholes_<kind>) or
+    None when it was not built."""
+    p = os.path.join(REF_DIR, f"blackscholes_{kind}")
+    return p if os.path.exists(p) else None

@@ -1,0 +1,435 @@
+// bs_capi.hip -- the C ABI of libbs_mi355.so (declared in include/bs_mi355.h).
+//
+// Argument checks, pointer validation and error mapping sit here; the kernels
+// are in bs_kernels.hip and the serving engine in bs_serve.hip.  Nothing in
+// this library prices on the CPU: every entry point that computes launches a
+// HIP kernel, and fails with BS_ERR_NODEVICE / BS_ERR_HIP when it cannot.
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "bs_internal.hpp"
+
+namespace bs {
+
+Config& config() {
+    static Config c;
+    return c;
+}
+
+static thread_local std::string t_last_error;
+
+int set_error(int status, const std::string& msg) {
+    t_last_error = msg;
+    return status;
+}
+
+int hip_error(hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" +
+                    hipGetErrorString(e) + ")";
+    if (e == hipErrorOutOfMemory) return set_error(BS_ERR_NOMEM, m);
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return set_error(BS_ERR_NODEVICE, m);
+    return set_error(BS_ERR_HIP, m);
+}
+
+int check_device_range(const void* p, size_t bytes, const char* name) {
+    if (p == nullptr) return set_error(BS_ERR_INVALID, std::string(name) + " is NULL");
+    if (!config().check_pointers) return BS_OK;
+    hipPointerAttribute_t attr;
+    memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(BS_ERR_INVALID, std::string(name) + " is not memory the GPU can access");
+    }
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged &&
+        !(attr.type == hipMemoryTypeHost && attr.devicePointer != nullptr))
+        return set_error(BS_ERR_INVALID,
+                         std::string(name) + " is not device, managed or pinned memory");
+    if (attr.type == hipMemoryTypeDevice) {
+        hipDeviceptr_t base = nullptr;
+        size_t size = 0;
+        e = hipMemGetAddressRange(&base, &size, const_cast<void*>(p));
+        if (e == hipSuccess) {
+            const char* b = static_cast<const char*>(base);
+            const char* q = static_cast<const char*>(p);
+            if (q + bytes > b + size)
+                return set_error(BS_ERR_INVALID,
+                                 std::string(name) + " allocation is smaller than n elements");
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    return BS_OK;
+}
+
+}  // namespace bs
+
+using namespace bs;
+
+namespace {
+
+hipStream_t as_stream(bs_stream_t s) { return static_cast<hipStream_t>(s); }
+
+int check_n(int64_t n) {
+    if (n < 0) return set_error(BS_ERR_INVALID, "n < 0");
+    return BS_OK;
+}
+
+template <class T>
+int check_soa(const T* s, const T* k, const T* r, const T* v, const T* t, int64_t n) {
+    const size_t b = (size_t)n * sizeof(T);
+    int rc;
+    if ((rc = check_device_range(s, b, "s")) || (rc = check_device_range(k, b, "k")) ||
+        (rc = check_device_range(r, b, "r")) || (rc = check_device_range(v, b, "v")) ||
+        (rc = check_device_range(t, b, "t")))
+        return rc;
+    return BS_OK;
+}
+
+int done(hipError_t e, const char* what) { return e == hipSuccess ? BS_OK : hip_error(e, what); }
+
+// Per-thread pinned word used by the scalar convenience forms.
+float* scalar_slot() {
+    static thread_local float* slot = nullptr;
+    if (slot == nullptr) {
+        if (hipHostMalloc((void**)&slot, sizeof(float) * 4, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            slot = nullptr;
+        }
+    }
+    return slot;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bs_status_string(int status) {
+    switch (status) {
+        case BS_OK: return "ok";
+        case BS_ERR_INVALID: return "invalid argument";
+        case BS_ERR_HIP: return "HIP runtime error";
+        case BS_ERR_NOMEM: return "out of device memory";
+        case BS_ERR_BUDGET: return "over the HBM budget";
+        case BS_ERR_NOT_FOUND: return "unknown book or ticket";
+        case BS_ERR_CANCELLED: return "cancelled";
+        case BS_ERR_TIMEOUT: return "timed out";
+        case BS_ERR_BUSY: return "book in use";
+        case BS_ERR_NODEVICE: return "no HIP device";
+        case BS_ERR_FAILED: return "request failed";
+        case BS_ERR_SHUTDOWN: return "context shutting down";
+        default: return "unknown status";
+    }
+}
+
+const char* bs_last_error(void) { return t_last_error.c_str(); }
+
+int bs_version(void) { return 10000 * 1 + 100 * 0 + 0; }
+
+int bs_config_set(int key, int64_t value) {
+    switch (key) {
+        case BS_CFG_CHECK_POINTERS: config().check_pointers = value != 0; return BS_OK;
+        case BS_CFG_MAX_BLOCKS:
+            if (value < 0) return set_error(BS_ERR_INVALID, "max_blocks < 0");
+            config().max_blocks = value;
+            return BS_OK;
+        case BS_CFG_UNROLL:
+            if (value != 1 && value != 2 && value != 4)
+                return set_error(BS_ERR_INVALID, "unroll must be 1, 2 or 4");
+            config().unroll = (int)value;
+            return BS_OK;
+        case BS_CFG_NONTEMPORAL: config().nontemporal = value != 0; return BS_OK;
+        default: return set_error(BS_ERR_INVALID, "unknown or read-only config key");
+    }
+}
+
+int bs_config_get(int key, int64_t* value) {
+    if (!value) return set_error(BS_ERR_INVALID, "value is NULL");
+    switch (key) {
+        case BS_CFG_CHECK_POINTERS: *value = config().check_pointers; return BS_OK;
+        case BS_CFG_MAX_BLOCKS: *value = config().max_blocks; return BS_OK;
+        case BS_CFG_BLOCK_THREADS: *value = kThreads; return BS_OK;
+        case BS_CFG_UNROLL: *value = config().unroll; return BS_OK;
+        case BS_CFG_NONTEMPORAL: *value = config().nontemporal; return BS_OK;
+        default: return set_error(BS_ERR_INVALID, "unknown config key");
+    }
+}
+
+// ---- per-option functions -----------------------------------------------------
+int bs_cndf_f32(const float* x, float* out, int64_t n, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((rc = check_device_range(x, n * 4, "x")) || (rc = check_device_range(out, n * 4, "out")))
+        return rc;
+    return done(launch_cndf_f32(x, out, n, as_stream(stream)), "bs_cndf_f32");
+}
+
+int bs_cndf_f64(const double* x, double* out, int64_t n, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((rc = check_device_range(x, n * 8, "x")) || (rc = check_device_range(out, n * 8, "out")))
+        return rc;
+    return done(launch_cndf_f64(x, out, n, as_stream(stream)), "bs_cndf_f64");
+}
+
+int bs_cndf_scalar_f32(float x, float* out) {
+    if (!out) return set_error(BS_ERR_INVALID, "out is NULL");
+    float* slot = scalar_slot();
+    if (!slot) return set_error(BS_ERR_NOMEM, "pinned scalar slot");
+    hipError_t e = launch_cndf_scalar(x, slot, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_error(e, "bs_cndf_scalar_f32");
+    *out = *slot;
+    return BS_OK;
+}
+
+int bs_price_scalar_f32(float sptprice, float strike, float rate, float volatility, float time,
+                        int otype, float timet, float* out) {
+    (void)timet;  // unused by the reference as well (blackscholes.c:192)
+    if (!out) return set_error(BS_ERR_INVALID, "out is NULL");
+    float* slot = scalar_slot();
+    if (!slot) return set_error(BS_ERR_NOMEM, "pinned scalar slot");
+    hipError_t e =
+        launch_price_scalar(sptprice, strike, rate, volatility, time, otype != 0, slot, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_error(e, "bs_price_scalar_f32");
+    *out = *slot;
+    return BS_OK;
+}
+
+// ---- batch pricing --------------------------------------------------------------
+int bs_price_f32(const float* s, const float* k, const float* r, const float* v, const float* t,
+                 const uint8_t* otype, float* price, int64_t n, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((rc = check_soa(s, k, r, v, t, n)) || (rc = check_device_range(otype, n, "otype")) ||
+        (rc = check_device_range(price, n * 4, "price")))
+        return rc;
+    return done(launch_price_f32({s, k, r, v, t}, otype, price, n, as_stream(stream)),
+                "bs_price_f32");
+}
+
+int bs_price_i32type_f32(const float* s, const float* k, const float* r, const float* v,
+                         const float* t, const int32_t* otype, float* price, int64_t n,
+                         bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((rc = check_soa(s, k, r, v, t, n)) || (rc = check_device_range(otype, n * 4, "otype")) ||
+        (rc = check_device_range(price, n * 4, "price")))
+        return rc;
+    return done(launch_price_i32type_f32({s, k, r, v, t}, otype, price, n, as_stream(stream)),
+                "bs_price_i32type_f32");
+}
+
+int bs_price_fast_f32(const float* s, const float* k, const float* r, const float* v,
+                          const float* t, const uint8_t* otype, float* price, int64_t n,
+                          bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((rc = check_soa(s, k, r, v, t, n)) || (rc = check_device_range(otype, n, "otype")) ||
+        (rc = check_device_range(price, n * 4, "price")))
+        return rc;
+    return done(launch_price_fast_f32({s, k, r, v, t}, otype, price, n, as_stream(stream)),
+                "bs_price_fast_f32");
+}
+
+int bs_price_f64(const double* s, const double* k, const double* r, const double* v,
+                 const double* t, const uint8_t* otype, double* price, int64_t n,
+                 bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((rc = check_soa(s, k, r, v, t, n)) || (rc = check_device_range(otype, n, "otype")) ||
+        (rc = check_device_range(price, n * 8, "price")))
+        return rc;
+    return done(launch_price_f64({s, k, r, v, t}, otype, price, n, as_stream(stream)),
+                "bs_price_f64");
+}
+
+int bs_price_greeks_f32(const float* s, const float* k, const float* r, const float* v,
+                        const float* t, const uint8_t* otype, float* price, float* delta,
+                        float* vega, float* rho, float* theta, int64_t n, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    const size_t b = (size_t)n * 4;
+    if ((rc = check_soa(s, k, r, v, t, n)) || (rc = check_device_range(otype, n, "otype")) ||
+        (rc = check_device_range(price, b, "price")) ||
+        (rc = check_device_range(delta, b, "delta")) ||
+        (rc = check_device_range(vega, b, "vega")) || (rc = check_device_range(rho, b, "rho")) ||
+        (rc = check_device_range(theta, b, "theta")))
+        return rc;
+    return done(launch_greeks_f32({s, k, r, v, t}, otype, {price, delta, vega, rho, theta}, n,
+                                  as_stream(stream)),
+                "bs_price_greeks_f32");
+}
+
+int bs_repack_aos_f32(const void* aos, int64_t n, float* s, float* k, float* r, float* v,
+                      float* t, uint8_t* otype, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if ((reinterpret_cast<uintptr_t>(aos) & 3) != 0)
+        return set_error(BS_ERR_INVALID, "aos must be 4-byte aligned");
+    if ((rc = check_device_range(aos, (size_t)n * 36, "aos")) ||
+        (rc = check_soa<float>(s, k, r, v, t, n)) ||
+        (rc = check_device_range(otype, n, "otype")))
+        return rc;
+    return done(launch_repack_aos(aos, n, s, k, r, v, t, otype, as_stream(stream)),
+                "bs_repack_aos_f32");
+}
+
+// ---- host-pointer forms -------------------------------------------------------------
+int bs_price_host_f32(const float* s, const float* k, const float* r, const float* v,
+                      const float* t, const int32_t* otype, float* price, int64_t n,
+                      int passes, int device) {
+    int rc;
+    if ((rc = check_n(n)) || n == 0) return rc;
+    if (!s || !k || !r || !v || !t || !otype || !price)
+        return set_error(BS_ERR_INVALID, "NULL host array");
+    if (passes < 1) return set_error(BS_ERR_INVALID, "passes < 1");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_error(e, "hipSetDevice");
+    // One allocation: 5 float arrays + int32 types + prices, each 256-B aligned.
+    const size_t fb = ((size_t)n * 4 + 255) & ~(size_t)255;
+    char* base = nullptr;
+    e = hipMalloc((void**)&base, fb * 7);
+    if (e != hipSuccess) return hip_error(e, "hipMalloc");
+    float* d[5];
+    const float* h[5] = {s, k, r, v, t};
+    for (int i = 0; i < 5; ++i) d[i] = reinterpret_cast<float*>(base + fb * i);
+    int32_t* dty = reinterpret_cast<int32_t*>(base + fb * 5);
+    float* dout = reinterpret_cast<float*>(base + fb * 6);
+    for (int i = 0; i < 5 && e == hipSuccess; ++i)
+        e = hipMemcpy(d[i], h[i], (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dty, otype, (size_t)n * 4, hipMemcpyHostToDevice);
+    for (int p = 0; p < passes && e == hipSuccess; ++p)
+        e = launch_price_i32type_f32({d[0], d[1], d[2], d[3], d[4]}, dty, dout, n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(price, dout, (size_t)n * 4, hipMemcpyDeviceToHost);
+    hipError_t ef = hipFree(base);
+    if (e != hipSuccess) return hip_error(e, "bs_price_host_f32");
+    if (ef != hipSuccess) return hip_error(ef, "hipFree");
+    return BS_OK;
+}
+
+// ---- bs_thread drop-in -------------------------------------------------------------
+static struct {
+    std::mutex mu;
+    int num_options = 0, nthreads = 1;
+    float *sptprice = nullptr, *strike = nullptr, *rate = nullptr, *volatility = nullptr,
+          *otime = nullptr, *prices = nullptr;
+    int* otype = nullptr;
+} g_globals;
+
+int bs_bind_globals(int num_options, int nthreads, float* sptprice, float* strike, float* rate,
+                    float* volatility, float* otime, int* otype, float* prices) {
+    if (num_options < 0 || nthreads < 1)
+        return set_error(BS_ERR_INVALID, "num_options < 0 or nthreads < 1");
+    std::lock_guard<std::mutex> lk(g_globals.mu);
+    g_globals.num_options = num_options;
+    g_globals.nthreads = nthreads;
+    g_globals.sptprice = sptprice;
+    g_globals.strike = strike;
+    g_globals.rate = rate;
+    g_globals.volatility = volatility;
+    g_globals.otime = otime;
+    g_globals.otype = otype;
+    g_globals.prices = prices;
+    return BS_OK;
+}
+
+int bs_thread_compat(void* tid_ptr) {
+    constexpr int kNumRuns = 100;  // NUM_RUNS, blackscholes.c:87
+    if (!tid_ptr) return set_error(BS_ERR_INVALID, "tid_ptr is NULL");
+    const int tid = *static_cast<int*>(tid_ptr);
+    int n, nt;
+    float *s, *k, *r, *v, *t, *out;
+    int* ty;
+    {
+        std::lock_guard<std::mutex> lk(g_globals.mu);
+        n = g_globals.num_options;
+        nt = g_globals.nthreads;
+        s = g_globals.sptprice;
+        k = g_globals.strike;
+        r = g_globals.rate;
+        v = g_globals.volatility;
+        t = g_globals.otime;
+        ty = g_globals.otype;
+        out = g_globals.prices;
+    }
+    if (tid < 0 || tid >= nt) return set_error(BS_ERR_INVALID, "tid outside [0, nthreads)");
+    // Block of the static partition (blackscholes.c:613-614): N/nthreads
+    // options from tid*(N/nthreads); with one thread that is the whole book.
+    const int64_t len = n / nt;
+    const int64_t start = (int64_t)tid * len;
+    if (len == 0) return BS_OK;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return bs_price_host_f32(s + start, k + start, r + start, v + start, t + start, ty + start,
+                             out + start, len, kNumRuns, dev);
+}
+
+// ---- device plumbing -----------------------------------------------------------------
+int bs_device_count(int* count) {
+    if (!count) return set_error(BS_ERR_INVALID, "count is NULL");
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_error(e, "hipGetDeviceCount");
+    }
+    return BS_OK;
+}
+int bs_set_device(int device) { return done(hipSetDevice(device), "hipSetDevice"); }
+int bs_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return set_error(BS_ERR_INVALID, "ptr is NULL");
+    return done(hipMalloc(ptr, bytes), "hipMalloc");
+}
+int bs_free(void* ptr) { return done(hipFree(ptr), "hipFree"); }
+int bs_memcpy_h2d(void* dst, const void* src, size_t bytes, bs_stream_t stream) {
+    return done(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)),
+                "hipMemcpyAsync H2D");
+}
+int bs_memcpy_d2h(void* dst, const void* src, size_t bytes, bs_stream_t stream) {
+    return done(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)),
+                "hipMemcpyAsync D2H");
+}
+int bs_memset(void* dst, int value, size_t bytes, bs_stream_t stream) {
+    return done(hipMemsetAsync(dst, value, bytes, as_stream(stream)), "hipMemsetAsync");
+}
+int bs_stream_create(bs_stream_t* stream) {
+    if (!stream) return set_error(BS_ERR_INVALID, "stream is NULL");
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    *stream = s;
+    return done(e, "hipStreamCreate");
+}
+int bs_stream_destroy(bs_stream_t stream) {
+    return done(hipStreamDestroy(as_stream(stream)), "hipStreamDestroy");
+}
+int bs_stream_sync(bs_stream_t stream) {
+    return done(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+}
+int bs_device_sync(void) { return done(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+int bs_event_create(bs_event_t* ev) {
+    if (!ev) return set_error(BS_ERR_INVALID, "ev is NULL");
+    hipEvent_t e = nullptr;
+    hipError_t rc = hipEventCreate(&e);
+    *ev = e;
+    return done(rc, "hipEventCreate");
+}
+int bs_event_destroy(bs_event_t ev) {
+    return done(hipEventDestroy(static_cast<hipEvent_t>(ev)), "hipEventDestroy");
+}
+int bs_event_record(bs_event_t ev, bs_stream_t stream) {
+    return done(hipEventRecord(static_cast<hipEvent_t>(ev), as_stream(stream)), "hipEventRecord");
+}
+int bs_event_sync(bs_event_t ev) {
+    return done(hipEventSynchronize(static_cast<hipEvent_t>(ev)), "hipEventSynchronize");
+}
+int bs_event_elapsed_ms(bs_event_t start, bs_event_t stop, float* ms) {
+    if (!ms) return set_error(BS_ERR_INVALID, "ms is NULL");
+    return done(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                                    static_cast<hipEvent_t>(stop)),
+                "hipEventElapsedTime");
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// bs_internal.hpp -- shared host-side declarations of libbs_mi355.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/bs_mi355.h"
+
+namespace bs {
+
+constexpr int kThreads = 256;  // threads per block: 4 wave64s
+
+struct Config {
+    int check_pointers = 1;
+    int64_t max_blocks = 0;  // 0: one 4-option group per thread, no grid cap
+    int unroll = 1;
+    int nontemporal = 1;
+};
+Config& config();
+
+// Thread-local error text behind bs_last_error().
+int set_error(int status, const std::string& msg);
+int hip_error(hipError_t e, const char* what);
+
+// Validates that [p, p + bytes) is memory the GPU may access.
+int check_device_range(const void* p, size_t bytes, const char* name);
+
+template <class T>
+struct SoaIn {
+    const T* s;
+    const T* k;
+    const T* r;
+    const T* v;
+    const T* t;
+};
+
+struct GreeksOut {
+    float* price;
+    float* delta;
+    float* vega;
+    float* rho;
+    float* theta;
+};
+
+// Kernel launchers (bs_kernels.hip).  They check nothing but alignment.
+hipError_t launch_price_f32(SoaIn<float> in, const uint8_t* type, float* out, int64_t n,
+                            hipStream_t st);
+hipError_t launch_price_i32type_f32(SoaIn<float> in, const int32_t* type, float* out,
+                                    int64_t n, hipStream_t st);
+hipError_t launch_price_fast_f32(SoaIn<float> in, const uint8_t* type, float* out,
+                                     int64_t n, hipStream_t st);
+hipError_t launch_price_f64(SoaIn<double> in, const uint8_t* type, double* out,
+                            int64_t n, hipStream_t st);
+hipError_t launch_greeks_f32(SoaIn<float> in, const uint8_t* type, GreeksOut out,
+                             int64_t n, hipStream_t st);
+hipError_t launch_cndf_f32(const float* x, float* out, int64_t n, hipStream_t st);
+hipError_t launch_cndf_f64(const double* x, double* out, int64_t n, hipStream_t st);
+hipError_t launch_repack_aos(const void* aos, int64_t n, float* s, float* k, float* r,
+                             float* v, float* t, uint8_t* type, hipStream_t st);
+hipError_t launch_price_scalar(float s, float k, float r, float v, float t, int put,
+                               float* out, hipStream_t st);
+hipError_t launch_cndf_scalar(float x, float* out, hipStream_t st);
+
+}  // namespace bs

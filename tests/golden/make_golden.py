@@ -1,0 +1,111 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Run in a tree where /root/reference exists (the reference is never needed at
+test time):
+
+    make -C oracle            # builds liboracle.so and oracle/_ref/*
+    python tests/golden/make_golden.py
+
+For each book it writes the PARSEC text input, runs the compiled reference
+(oracle/_ref/blackscholes_serial 1 in out, and blackscholes_ff with 8
+threads), reads the "%.18f" price lines, and stores inputs plus reference
+prices as an .npz fixture.  It also checks that
+
+  * the FastFlow build's output text is identical to the serial build's, and
+  * the oracle restatement (oracle/bs_oracle.c) prints exactly the same
+    "%.18f" text as the reference, option by option,
+
+on the fixture books and on a 1M-option book that is checked but not stored.
+A summary goes to tests/golden/manifest.json.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from p3arsec_amd import books  # noqa: E402
+import pyoracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_reference(book, tmp, tag, kind="serial", threads=1):
+    exe = pyoracle.ref_binary(kind)
+    if exe is None:
+        raise SystemExit(f"compiled reference {kind} missing: run make -C oracle ref")
+    inp = os.path.join(tmp, f"{tag}.in")
+    out = os.path.join(tmp, f"{tag}.{kind}.out")
+    if not os.path.exists(inp):
+        books.write_parsec_input(inp, book)
+    subprocess.run([exe, str(threads), inp, out], check=True, stdout=subprocess.DEVNULL)
+    with open(out) as f:
+        text = f.read()
+    return text
+
+
+def oracle_text(prices):
+    return "%d\n" % prices.shape[0] + "".join("%.18f\n" % float(p) for p in prices)
+
+
+def check_set(book, tmp, tag):
+    serial = run_reference(book, tmp, tag, "serial", 1)
+    ff = run_reference(book, tmp, tag, "ff", 8)
+    mine, _, _ = pyoracle.price_f32(book, nthreads=4, passes=1)
+    mine_text = oracle_text(mine)
+    ref_lines = serial.splitlines()[1:]
+    my_lines = mine_text.splitlines()[1:]
+    mismatch = sum(1 for a, b in zip(ref_lines, my_lines) if a != b)
+    info = {
+        "n": book.n,
+        "ff_equals_serial": ff == serial,
+        "oracle_text_mismatches": mismatch,
+        "reference_sha256": hashlib.sha256(serial.encode()).hexdigest(),
+    }
+    ref_price = np.array([float(x) for x in ref_lines], dtype=np.float64).astype(np.float32)
+    return ref_price, info
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    manifest = {"generator": "tests/golden/make_golden.py",
+                "reference": "parsec-ff/pkgs/apps/blackscholes/src/blackscholes.c "
+                             "(serial and -DENABLE_FF builds, oracle/Makefile)",
+                "sets": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        sets = {
+            "config1_4096": books.make_book(4096, books.SEED_BASE + 1),
+            "edge": books.edge_book(),
+        }
+        for tag, book in sets.items():
+            ref_price, info = check_set(book, tmp, tag)
+            np.savez_compressed(os.path.join(OUT, f"golden_{tag}.npz"),
+                                s=book.s, k=book.k, r=book.r, v=book.v, t=book.t,
+                                otype=book.otype, ref_price=ref_price)
+            manifest["sets"][tag] = info
+            print(tag, info)
+        big = books.make_book(1 << 20, books.SEED_BASE + 2)
+        _, info = check_set(big, tmp, "check_1m")
+        info["stored"] = False
+        manifest["sets"]["check_1m"] = info
+        print("check_1m", info)
+    with open(os.path.join(OUT, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+        f.write("\n")
+    bad = [t for t, i in manifest["sets"].items()
+           if i["oracle_text_mismatches"] or not i["ff_equals_serial"]]
+    if bad:
+        raise SystemExit(f"oracle does not reproduce the reference on {bad}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+"""The C ABI library without a GPU: it loads, exports every function the
+header declares, and rejects bad arguments before any device work."""
+import ctypes
+import subprocess
+
+import numpy as np
+
+from p3arsec_amd import capi
+
+
+def test_exports_every_declared_function():
+    L = capi.load()
+    declared = capi.header_functions()
+    assert len(declared) >= 40
+    missing = [f for f in declared if not hasattr(L, f)]
+    assert not missing, missing
+    # and the binding knows the signature of each of them
+    assert sorted(capi._SIGS) == declared
+
+
+def test_exported_symbols_are_c_abi():
+    out = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], check=True,
+                         capture_output=True, text=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for f in capi.header_functions():
+        assert f in names, f          # unmangled extern "C" names
+
+
+def test_status_strings():
+    L = capi.load()
+    for code in capi.STATUS:
+        assert L.bs_status_string(code)
+    assert L.bs_status_string(12345) == b"unknown status"
+    assert L.bs_version() >= 10000
+
+
+def test_config_roundtrip_and_validation():
+    L = capi.load()
+    old = capi.get_config(capi.CFG_UNROLL)
+    capi.set_config(capi.CFG_UNROLL, 2)
+    assert capi.get_config(capi.CFG_UNROLL) == 2
+    capi.set_config(capi.CFG_UNROLL, old)
+    assert L.bs_config_set(capi.CFG_UNROLL, 3) == -1
+    assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 64) == -1   # read-only
+    assert L.bs_config_set(999, 1) == -1
+    assert capi.get_config(capi.CFG_BLOCK_THREADS) == 256
+
+
+def test_argument_errors_before_device_work():
+    L = capi.load()
+    nul = [None] * 7
+    assert L.bs_price_f32(*nul, -1, None) == -1           # n < 0
+    assert L.bs_price_f32(*nul, 0, None) == 0             # empty map: nothing to do
+    assert L.bs_price_f32(*nul, 16, None) == -1           # NULL arrays
+    assert b"NULL" in L.bs_last_error()
+    assert L.bs_price_greeks_f32(*([None] * 11), 0, None) == 0
+    assert L.bs_price_host_f32(*nul, 8, 1, 0) == -1
+    host = np.zeros(16, np.float32)
+    p = host.ctypes.data
+    # plain host memory is never handed to a kernel
+    assert L.bs_price_f32(p, p, p, p, p, p, p, 16, None) == -1
+    assert L.bs_repack_aos_f32(p + 1, 4, p, p, p, p, p, p, None) == -1   # misaligned AoS
+
+
+def test_scalar_and_thread_compat_arguments():
+    L = capi.load()
+    assert L.bs_price_scalar_f32(1, 1, 0.1, 0.2, 1, 0, 0, None) == -1
+    assert L.bs_thread_compat(None) == -1
+    assert L.bs_bind_globals(-1, 1, *([None] * 7)) == -1
+    assert L.bs_bind_globals(10, 0, *([None] * 7)) == -1
+    tid = ctypes.c_int(5)
+    assert L.bs_bind_globals(10, 2, *([None] * 7)) == 0
+    assert L.bs_thread_compat(ctypes.byref(tid)) == -1       # tid >= nthreads

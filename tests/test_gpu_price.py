@@ -1,0 +1,218 @@
+"""Parity of the HIP pricing kernels with the oracle (runs on the MI355X).
+
+Bar: every FP32 price within 1e-4 of the reference's own output (the
+reference's ERR_CHK threshold, blackscholes.c:333-340); the fptype=double map
+within 1e-9 of the FP64 restatement; the launch shape (alignment, grid cap,
+unroll, non-temporal) never changes a bit.  Every test calls through the C ABI.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from p3arsec_amd import books
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+# bs_price_fast_f32 (hardware transcendentals) is documented to exceed the
+# reference's check on a handful of options per 10M; it is held to 2e-4.
+TOL_FAST = 2e-4
+
+
+def run(capi, book, mode="ref", **kw):
+    dt = np.float64 if mode == "f64" else np.float32
+    tdt = np.int32 if mode == "i32" else np.uint8
+    db = capi.DeviceBook(book, dtype=dt, type_dtype=tdt, pad=kw.pop("pad", 0))
+    out_pad = kw.pop("out_pad", 0)
+    out = capi.price(db, mode=mode, out_pad=out_pad)
+    got = out.download(dt, book.n, offset_bytes=out_pad * np.dtype(dt).itemsize)
+    db.free()
+    out.free()
+    return got
+
+
+def err_stats(got, ref):
+    e = np.abs(got.astype(np.float64) - ref.astype(np.float64))
+    return float(e.max()) if e.size else 0.0, float(e.mean()) if e.size else 0.0
+
+
+@pytest.mark.parametrize("tag", ["config1_4096", "edge"])
+@pytest.mark.parametrize("mode", ["ref", "i32", "fast"])
+def test_golden_vectors(gpu, oracle, tag, mode):
+    book, ref = load_golden(tag)
+    got = run(gpu, book, mode)
+    mx, mean = err_stats(got, ref)
+    exact = oracle.price_exact(book)
+    mx64, mean64 = err_stats(got, exact)
+    same = float(np.mean(got.view(np.uint32) == ref.view(np.uint32)))
+    print(f"\n{tag} {mode}: vs reference max {mx:.3e} mean {mean:.3e} "
+          f"bit-identical {same:.4f}; vs FP64 closed form max {mx64:.3e} mean {mean64:.3e}")
+    assert mx <= (TOL_FAST if mode == "fast" else TOL)
+
+
+def test_random_book_1m(gpu, oracle):
+    book = books.make_book(1 << 20, books.SEED_BASE + 21)
+    ref, _, _ = oracle.price_f32(book)
+    for mode in ("ref", "fast"):
+        got = run(gpu, book, mode)
+        mx, mean = err_stats(got, ref)
+        print(f"\n1M {mode}: max {mx:.3e} mean {mean:.3e} "
+              f"bit-identical {np.mean(got == ref):.4f}")
+        assert mx <= (TOL_FAST if mode == "fast" else TOL)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 255, 256, 257, 1023, 1025, 4099])
+def test_ragged_sizes(gpu, oracle, n):
+    book = books.make_book(n, books.SEED_BASE + 22 + n)
+    ref, _, _ = oracle.price_f32(book)
+    got = run(gpu, book)
+    assert err_stats(got, ref)[0] <= TOL
+
+
+def test_empty_book(gpu):
+    L = gpu.load()
+    assert L.bs_price_f32(*([None] * 7), 0, None) == 0
+
+
+def test_launch_shape_and_alignment_do_not_change_bits(gpu):
+    book = books.make_book(100_003, books.SEED_BASE + 23)
+    base = run(gpu, book)
+    variants = [run(gpu, book, pad=1), run(gpu, book, out_pad=1),   # scalar path
+                run(gpu, book, "i32")]
+    try:
+        for key, val in ((gpu.CFG_MAX_BLOCKS, 7), (gpu.CFG_UNROLL, 2), (gpu.CFG_UNROLL, 4),
+                         (gpu.CFG_NONTEMPORAL, 0)):
+            gpu.set_config(key, val)
+            variants.append(run(gpu, book))
+            gpu.set_config(gpu.CFG_MAX_BLOCKS, 0)
+            gpu.set_config(gpu.CFG_UNROLL, 1)
+            gpu.set_config(gpu.CFG_NONTEMPORAL, 1)
+    finally:
+        gpu.set_config(gpu.CFG_MAX_BLOCKS, 0)
+        gpu.set_config(gpu.CFG_UNROLL, 1)
+        gpu.set_config(gpu.CFG_NONTEMPORAL, 1)
+    for v in variants:
+        assert np.array_equal(v.view(np.uint32), base.view(np.uint32))
+
+
+def test_full_size_10m_against_oracle_and_parity(gpu, oracle):
+    n = 10_000_000
+    book = books.make_book(n, books.SEED_BASE + 2)
+    got = run(gpu, book)
+    ref, _, _ = oracle.price_f32(book)
+    mx, mean = err_stats(got, ref)
+    print(f"\n10M vs oracle: max {mx:.3e} mean {mean:.3e} "
+          f"bit-identical {np.mean(got == ref):.5f}")
+    fast = run(gpu, book, "fast")
+    fmx, fmean = err_stats(fast, ref)
+    print(f"10M fast vs oracle: max {fmx:.3e} mean {fmean:.3e} "
+          f"over 1e-4: {int(np.sum(np.abs(fast.astype(np.float64) - ref) > 1e-4))}")
+    assert mx <= TOL and fmx <= TOL_FAST
+    # size-independent check: put-call parity across a call book and a put book
+    calls = run(gpu, books.Book(*book.arrays()[:5], np.zeros(n, np.uint8)))
+    puts = run(gpu, books.Book(*book.arrays()[:5], np.ones(n, np.uint8)))
+    s, k, r, t = (a.astype(np.float64) for a in (book.s, book.k, book.r, book.t))
+    fwd = s - k * np.exp(-r * t)
+    assert np.max(np.abs(calls - puts - fwd)) < 2 * TOL
+
+
+def test_fp64_map(gpu, oracle):
+    book = books.make_book(1 << 20, books.SEED_BASE + 3)
+    got = run(gpu, book, "f64")
+    ref = oracle.price_f64(book.astype(np.float64))
+    mx, mean = err_stats(got, ref)
+    f32 = run(gpu, book)
+    d32 = err_stats(f32, got)
+    ex = err_stats(got, oracle.price_exact(book))
+    print(f"\nFP64 vs FP64 oracle max {mx:.3e}; vs closed form max {ex[0]:.3e} mean {ex[1]:.3e};"
+          f" FP32 - FP64 max {d32[0]:.3e} mean {d32[1]:.3e}")
+    assert mx <= 1e-9
+    s, k = book.s.astype(np.float64), book.k.astype(np.float64)
+    assert np.all(np.abs(got - oracle.price_exact(book)) <= 7.5e-8 * (s + k) + 1e-9)
+
+
+def test_cndf_batch(gpu, oracle):
+    x = np.linspace(-9, 9, 100_003)
+    db = gpu.DeviceBuffer.from_array(x.astype(np.float32))
+    out = gpu.DeviceBuffer(x.size * 4)
+    gpu.call("bs_cndf_f32", db.ptr, out.ptr, x.size, None)
+    got = out.download(np.float32, x.size)
+    ref = oracle.cndf_f32(x.astype(np.float32))
+    assert np.max(np.abs(got.astype(np.float64) - ref)) < 3e-7
+    d64 = gpu.DeviceBuffer.from_array(x)
+    o64 = gpu.DeviceBuffer(x.size * 8)
+    gpu.call("bs_cndf_f64", d64.ptr, o64.ptr, x.size, None)
+    assert np.max(np.abs(o64.download(np.float64, x.size) - oracle.cndf_f64(x))) < 1e-15
+
+
+def test_scalar_entry_points(gpu, oracle):
+    L = gpu.load()
+    out = ctypes.c_float()
+    gpu.check(L.bs_price_scalar_f32(42.0, 40.0, 0.1, 0.2, 0.5, 0, 0.0, ctypes.byref(out)), "s")
+    assert abs(out.value - 4.7594) < 1e-4
+    gpu.check(L.bs_price_scalar_f32(42.0, 40.0, 0.1, 0.2, 0.5, 1, 0.0, ctypes.byref(out)), "s")
+    assert abs(out.value - 0.8086) < 1e-4
+    gpu.check(L.bs_cndf_scalar_f32(0.0, ctypes.byref(out)), "c")
+    assert abs(out.value - 0.5) < 1e-6
+
+
+def test_host_pointer_form_and_thread_compat(gpu, oracle):
+    book = books.make_book(10_007, books.SEED_BASE + 24)
+    ref, _, _ = oracle.price_f32(book)
+    arrs = [np.ascontiguousarray(a) for a in (book.s, book.k, book.r, book.v, book.t)]
+    ty = book.otype.astype(np.int32)
+    out = np.zeros(book.n, np.float32)
+    L = gpu.load()
+    gpu.check(L.bs_price_host_f32(*[a.ctypes.data for a in arrs], ty.ctypes.data,
+                                  out.ctypes.data, book.n, 3, 0), "host")
+    assert err_stats(out, ref)[0] <= TOL
+    # bs_thread drop-in: 3 threads, static blocks of N/3, the N%3 tail untouched
+    prices = np.full(book.n, -1.0, np.float32)
+    gpu.check(L.bs_bind_globals(book.n, 3, *[a.ctypes.data for a in arrs], ty.ctypes.data,
+                                prices.ctypes.data), "bind")
+    for tid in range(3):
+        t = ctypes.c_int(tid)
+        gpu.check(L.bs_thread_compat(ctypes.byref(t)), "compat")
+    blk = book.n // 3
+    assert err_stats(prices[:3 * blk], ref[:3 * blk])[0] <= TOL
+    assert np.all(prices[3 * blk:] == -1.0)
+
+
+def test_repack_aos(gpu):
+    n = 5000
+    book = books.make_book(n, books.SEED_BASE + 25)
+    rec = np.dtype([("s", "<f4"), ("strike", "<f4"), ("r", "<f4"), ("divq", "<f4"),
+                    ("v", "<f4"), ("t", "<f4"), ("OptionType", "S1"), ("pad", "V3"),
+                    ("divs", "<f4"), ("DGrefval", "<f4")])
+    assert rec.itemsize == 36
+    aos = np.zeros(n, rec)
+    for f, a in (("s", book.s), ("strike", book.k), ("r", book.r), ("v", book.v),
+                 ("t", book.t)):
+        aos[f] = a
+    chars = np.array([b"P", b"C", b"X", b"p"])
+    pick = np.arange(n) % 4
+    aos["OptionType"] = chars[pick]
+    aos["divq"] = 7.0
+    daos = gpu.DeviceBuffer.from_array(aos.view(np.uint8))
+    outs = [gpu.DeviceBuffer(n * 4) for _ in range(5)]
+    dty = gpu.DeviceBuffer(n)
+    gpu.call("bs_repack_aos_f32", daos.ptr, n, *[o.ptr for o in outs], dty.ptr, None)
+    gpu.call("bs_stream_sync", None)
+    for o, a in zip(outs, (book.s, book.k, book.r, book.v, book.t)):
+        assert np.array_equal(o.download(np.float32, n), a)
+    assert np.array_equal(dty.download(np.uint8, n), (pick == 0).astype(np.uint8))
+
+
+def test_invalid_pointers_rejected_on_device(gpu):
+    L = gpu.load()
+    small = gpu.DeviceBuffer(64)
+    big = gpu.DeviceBuffer(4096)
+    host = np.zeros(1024, np.float32)
+    p = [big.ptr] * 5 + [big.ptr, big.ptr]
+    # allocation shorter than n elements
+    assert L.bs_price_f32(*p[:4], small.ptr, big.ptr, big.ptr, 1000, None) == -1
+    # unpinned host memory
+    assert L.bs_price_f32(*p[:6], host.ctypes.data, 1000, None) == -1
+    gpu.call("bs_price_f32", *p, 1000, None)
+    gpu.call("bs_stream_sync", None)
