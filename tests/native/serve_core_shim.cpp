@@ -1,0 +1,87 @@
+// Test-only C front of the serving loop's host bookkeeping (serve_core), so
+// the CPU tests can drive admission, queueing, cancellation and failure paths
+// without a GPU.  Built by tests/native/Makefile; never part of the product.
+#include <stdint.h>
+
+#include "../../p3arsec_amd/csrc/serve_core.hpp"
+
+using bs::serve::Core;
+
+extern "C" {
+
+void* sc_create(int64_t arena_bytes, int64_t max_req, int64_t max_opts) {
+    bs::serve::Limits lim;
+    if (max_req > 0) lim.max_batch_requests = max_req;
+    if (max_opts > 0) lim.max_batch_options = max_opts;
+    return new Core((size_t)arena_bytes, lim);
+}
+void sc_destroy(void* h) { delete static_cast<Core*>(h); }
+
+int sc_register(void* h, int64_t n, int64_t* id, int64_t* off) {
+    size_t o = 0;
+    int rc = static_cast<Core*>(h)->register_book(n, id, &o);
+    *off = (int64_t)o;
+    return rc;
+}
+int sc_release_book(void* h, int64_t id) { return static_cast<Core*>(h)->release_book(id); }
+int sc_submit(void* h, int64_t book, int kind, int64_t now, int64_t* ticket) {
+    return static_cast<Core*>(h)->submit(book, kind, now, ticket);
+}
+int sc_cancel(void* h, int64_t t, int64_t now) { return static_cast<Core*>(h)->cancel(t, now); }
+// Admits a batch; writes up to cap tickets and output offsets, returns the count.
+int sc_admit(void* h, int64_t now, int64_t* tickets, int64_t* offs, int cap) {
+    auto b = static_cast<Core*>(h)->admit(now);
+    int i = 0;
+    for (auto* r : b) {
+        if (i < cap) {
+            tickets[i] = r->ticket;
+            offs[i] = (int64_t)r->out_off;
+        }
+        ++i;
+    }
+    return i;
+}
+void sc_complete(void* h, const int64_t* tickets, int n, int ok, int64_t now) {
+    std::vector<int64_t> v(tickets, tickets + n);
+    static_cast<Core*>(h)->complete(v, ok != 0, now);
+}
+void sc_fail_queued(void* h, int status, int64_t now) {
+    static_cast<Core*>(h)->fail_queued(status, now);
+}
+int sc_release_result(void* h, int64_t t) { return static_cast<Core*>(h)->release_result(t); }
+int sc_state(void* h, int64_t t, int* state, int* status) {
+    auto* r = static_cast<Core*>(h)->request(t);
+    if (!r) return bs::serve::kErrNotFound;
+    *state = r->state;
+    *status = r->status;
+    return 0;
+}
+int64_t sc_arena_used(void* h) { return (int64_t)static_cast<Core*>(h)->arena().used(); }
+int64_t sc_arena_peak(void* h) { return (int64_t)static_cast<Core*>(h)->arena().peak(); }
+int64_t sc_arena_largest_free(void* h) {
+    return (int64_t)static_cast<Core*>(h)->arena().largest_free();
+}
+int64_t sc_queued(void* h) { return (int64_t)static_cast<Core*>(h)->queued(); }
+void sc_stats(void* h, int64_t* out9) {
+    auto s = static_cast<Core*>(h)->stats();
+    int64_t v[9] = {s.books_resident, s.requests_queued, s.requests_running, s.requests_done,
+                    s.requests_failed, s.requests_cancelled, s.launches, s.options_priced,
+                    s.admission_waits};
+    for (int i = 0; i < 9; ++i) out9[i] = v[i];
+}
+int64_t sc_book_bytes(int64_t n) { return (int64_t)bs::serve::book_bytes(n); }
+int64_t sc_output_bytes(int64_t n, int kind) { return (int64_t)bs::serve::output_bytes(n, kind); }
+
+// Arena alone, for allocator property tests.
+void* sa_create(int64_t cap) { return new bs::serve::Arena((size_t)cap); }
+void sa_destroy(void* a) { delete static_cast<bs::serve::Arena*>(a); }
+int sa_alloc(void* a, int64_t bytes, int64_t* off) {
+    size_t o = 0;
+    bool ok = static_cast<bs::serve::Arena*>(a)->alloc((size_t)bytes, &o);
+    *off = (int64_t)o;
+    return ok ? 1 : 0;
+}
+int sa_free(void* a, int64_t off) { return static_cast<bs::serve::Arena*>(a)->free((size_t)off) ? 1 : 0; }
+int64_t sa_used(void* a) { return (int64_t)static_cast<bs::serve::Arena*>(a)->used(); }
+int64_t sa_largest_free(void* a) { return (int64_t)static_cast<bs::serve::Arena*>(a)->largest_free(); }
+}

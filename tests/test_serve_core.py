@@ -1,0 +1,239 @@
+"""The serving loop's host bookkeeping on the CPU (no GPU): the HBM arena
+allocator, FIFO admission by memory, batch limits, cancellation, failure
+injection and shutdown.  Drives p3arsec_amd/csrc/serve_core.cpp through the
+test-only C front in tests/native/serve_core_shim.cpp."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NATIVE = os.path.join(HERE, "native")
+OK, INVALID, BUDGET, NOT_FOUND, CANCELLED, BUSY, FAILED, SHUTDOWN = 0, -1, -4, -5, -6, -8, -10, -11
+QUEUED, RUNNING, DONE, FAILED_ST, CANCELLED_ST = range(5)
+PRICE, GREEKS = 1, 2
+i64 = ctypes.c_int64
+
+
+@pytest.fixture(scope="module")
+def L():
+    subprocess.run(["make", "-s", "-C", NATIVE], check=True)
+    lib = ctypes.CDLL(os.path.join(NATIVE, "libserve_core_shim.so"))
+    vp = ctypes.c_void_p
+    P = ctypes.POINTER
+    sig = {
+        "sc_create": ([i64, i64, i64], vp), "sc_destroy": ([vp], None),
+        "sc_register": ([vp, i64, P(i64), P(i64)], ctypes.c_int),
+        "sc_release_book": ([vp, i64], ctypes.c_int),
+        "sc_submit": ([vp, i64, ctypes.c_int, i64, P(i64)], ctypes.c_int),
+        "sc_cancel": ([vp, i64, i64], ctypes.c_int),
+        "sc_admit": ([vp, i64, P(i64), P(i64), ctypes.c_int], ctypes.c_int),
+        "sc_complete": ([vp, P(i64), ctypes.c_int, ctypes.c_int, i64], None),
+        "sc_fail_queued": ([vp, ctypes.c_int, i64], None),
+        "sc_release_result": ([vp, i64], ctypes.c_int),
+        "sc_state": ([vp, i64, P(ctypes.c_int), P(ctypes.c_int)], ctypes.c_int),
+        "sc_arena_used": ([vp], i64), "sc_arena_peak": ([vp], i64),
+        "sc_arena_largest_free": ([vp], i64), "sc_queued": ([vp], i64),
+        "sc_stats": ([vp, P(i64)], None),
+        "sc_book_bytes": ([i64], i64), "sc_output_bytes": ([i64, ctypes.c_int], i64),
+        "sa_create": ([i64], vp), "sa_destroy": ([vp], None),
+        "sa_alloc": ([vp, i64, P(i64)], ctypes.c_int), "sa_free": ([vp, i64], ctypes.c_int),
+        "sa_used": ([vp], i64), "sa_largest_free": ([vp], i64),
+    }
+    for name, (a, r) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = a
+        f.restype = r
+    return lib
+
+
+class Core:
+    def __init__(self, L, arena, max_req=0, max_opts=0):
+        self.L = L
+        self.h = L.sc_create(arena, max_req, max_opts)
+        self.clock = 0
+
+    def __del__(self):
+        self.L.sc_destroy(self.h)
+
+    def book(self, n):
+        i, o = i64(), i64()
+        rc = self.L.sc_register(self.h, n, ctypes.byref(i), ctypes.byref(o))
+        return rc, i.value, o.value
+
+    def submit(self, book, kind=PRICE):
+        t = i64()
+        self.clock += 1
+        rc = self.L.sc_submit(self.h, book, kind, self.clock, ctypes.byref(t))
+        return rc, t.value
+
+    def admit(self, cap=8192):
+        t = (i64 * cap)()
+        o = (i64 * cap)()
+        self.clock += 1
+        k = self.L.sc_admit(self.h, self.clock, t, o, cap)
+        return list(t[:k]), list(o[:k])
+
+    def complete(self, tickets, ok=True):
+        arr = (i64 * max(1, len(tickets)))(*tickets)
+        self.clock += 1
+        self.L.sc_complete(self.h, arr, len(tickets), 1 if ok else 0, self.clock)
+
+    def state(self, t):
+        s, st = ctypes.c_int(), ctypes.c_int()
+        rc = self.L.sc_state(self.h, t, ctypes.byref(s), ctypes.byref(st))
+        return rc, s.value, st.value
+
+    def stats(self):
+        out = (i64 * 9)()
+        self.L.sc_stats(self.h, out)
+        keys = ("books", "queued", "running", "done", "failed", "cancelled", "launches",
+                "options", "waits")
+        return dict(zip(keys, out))
+
+
+def test_layout_sizes(L):
+    assert L.sc_book_bytes(1) == 6 * 256
+    assert L.sc_book_bytes(1000) == 5 * 4096 + 1024
+    assert L.sc_output_bytes(1000, PRICE) == 4096
+    assert L.sc_output_bytes(1000, GREEKS) == 5 * 4096
+
+
+def test_arena_against_a_model(L):
+    """Random alloc/free against an explicit interval model: no overlap, every
+    block inside the arena, used bytes exact, full coalescing when empty."""
+    cap = 1 << 20
+    a = L.sa_create(cap)
+    rng = np.random.default_rng(5)
+    live = {}
+    for step in range(4000):
+        if live and rng.random() < 0.45:
+            off = int(rng.choice(list(live)))
+            assert L.sa_free(a, off) == 1
+            del live[off]
+        else:
+            size = int(rng.integers(1, 40000))
+            o = i64()
+            ok = L.sa_alloc(a, size, ctypes.byref(o))
+            need = (size + 255) // 256 * 256
+            if ok:
+                off = o.value
+                assert off % 256 == 0 and off + need <= cap
+                for lo, sz in live.items():
+                    assert off + need <= lo or lo + sz <= off, "overlap"
+                live[off] = need
+            else:
+                assert L.sa_largest_free(a) < need
+        assert L.sa_used(a) == sum(live.values())
+    assert L.sa_free(a, 12345) == 0                     # not a live block
+    for off in list(live):
+        L.sa_free(a, off)
+    assert L.sa_used(a) == 0 and L.sa_largest_free(a) == cap
+    L.sa_destroy(a)
+
+
+def test_register_budget_and_release(L):
+    c = Core(L, 1 << 20)
+    rc, b1, _ = c.book(10_000)           # 5*40192 + 10240 bytes
+    assert rc == OK
+    assert c.book(100_000)[0] == BUDGET  # does not fit
+    assert c.book(0)[0] == INVALID
+    assert L.sc_release_book(c.h, 999) == NOT_FOUND
+    rc, t = c.submit(b1)
+    assert rc == OK
+    assert L.sc_release_book(c.h, b1) == BUSY       # referenced by a queued request
+    assert c.submit(b1, kind=7)[0] == INVALID
+    assert c.submit(424242)[0] == NOT_FOUND
+    tk, _ = c.admit()
+    c.complete(tk)
+    assert L.sc_release_book(c.h, b1) == OK
+
+
+def test_fifo_admission_waits_for_memory(L):
+    # arena: one book of 1000 options (21.5 KB) + room for ~3 price outputs
+    bb = L.sc_book_bytes(1000)
+    ob = L.sc_output_bytes(1000, PRICE)
+    c = Core(L, bb + 3 * ob)
+    _, b, _ = c.book(1000)
+    tickets = [c.submit(b)[1] for _ in range(5)]
+    first, offs = c.admit()
+    assert first == tickets[:3] and len(set(offs)) == 3
+    assert c.admit()[0] == []                        # no memory: the 4th waits
+    assert c.stats()["waits"] >= 1
+    assert c.state(tickets[3])[1] == QUEUED
+    c.complete(first)
+    assert c.admit()[0] == []                        # done but results not released
+    assert L.sc_release_result(c.h, tickets[0]) == OK
+    nxt, _ = c.admit()
+    assert nxt == [tickets[3]]                       # FIFO order kept
+    assert L.sc_release_result(c.h, tickets[1]) == OK
+    assert c.admit()[0] == [tickets[4]]
+
+
+def test_batch_limits(L):
+    c = Core(L, 64 << 20, max_req=4, max_opts=2500)
+    _, b, _ = c.book(1000)
+    ts = [c.submit(b)[1] for _ in range(10)]
+    got, _ = c.admit()
+    assert got == ts[:2]                              # 2 x 1000 <= 2500 < 3 x 1000
+    c2 = Core(L, 64 << 20, max_req=3)
+    _, b2, _ = c2.book(10)
+    ts2 = [c2.submit(b2)[1] for _ in range(7)]
+    assert c2.admit()[0] == ts2[:3] and c2.admit()[0] == ts2[3:6]
+
+
+def test_cancel_and_states(L):
+    c = Core(L, 1 << 20)
+    _, b, _ = c.book(100)
+    _, t1 = c.submit(b)
+    _, t2 = c.submit(b, GREEKS)
+    assert L.sc_cancel(c.h, t2, 0) == OK
+    assert c.state(t2)[1:] == (CANCELLED_ST, CANCELLED)
+    assert L.sc_cancel(c.h, t2, 0) == BUSY           # no longer queued
+    got, _ = c.admit()
+    assert got == [t1]
+    assert L.sc_cancel(c.h, t1, 0) == BUSY           # running requests finish
+    assert L.sc_release_result(c.h, t1) == BUSY
+    c.complete([t1])
+    assert c.state(t1)[1] == DONE
+    assert L.sc_release_result(c.h, t1) == OK
+    assert c.state(t1)[0] == NOT_FOUND
+    assert L.sc_release_result(c.h, t2) == OK
+
+
+def test_failed_batch_frees_outputs(L):
+    c = Core(L, 1 << 20)
+    _, b, _ = c.book(1000)
+    used0 = L.sc_arena_used(c.h)
+    ts = [c.submit(b, GREEKS)[1] for _ in range(3)]
+    got, _ = c.admit()
+    assert L.sc_arena_used(c.h) == used0 + 3 * L.sc_output_bytes(1000, GREEKS)
+    c.complete(got, ok=False)                        # injected launch failure
+    for t in ts:
+        assert c.state(t)[1:] == (FAILED_ST, FAILED)
+    assert L.sc_arena_used(c.h) == used0
+    assert L.sc_release_book(c.h, b) == OK           # no dangling references
+    s = c.stats()
+    assert s["failed"] == 3 and s["running"] == 0
+
+
+def test_shutdown_fails_queued(L):
+    c = Core(L, 1 << 20)
+    _, b, _ = c.book(10)
+    ts = [c.submit(b)[1] for _ in range(4)]
+    L.sc_fail_queued(c.h, SHUTDOWN, 99)
+    for t in ts:
+        assert c.state(t)[1:] == (FAILED_ST, SHUTDOWN)
+    assert L.sc_queued(c.h) == 0 and L.sc_release_book(c.h, b) == OK
+
+
+def test_never_fitting_request_is_refused(L):
+    # outputs that cannot fit next to their own book are refused at submit
+    bb = L.sc_book_bytes(300_000)
+    c = Core(L, bb + L.sc_output_bytes(300_000, PRICE))
+    rc, b, _ = c.book(300_000)
+    assert rc == OK
+    assert c.submit(b, PRICE)[0] == OK
+    assert c.submit(b, GREEKS)[0] == BUDGET
