@@ -74,6 +74,43 @@ def cpu_baseline(book_host, passes: int) -> dict:
                       f"blackscholes.c), {sec:.2f} s"}
 
 
+def only_one(args, L, host, dbooks, outs, stream) -> int:
+    """Runs one kernel for --warmup + --steps passes (profiler runs)."""
+    n, nb = host.n, len(dbooks)
+    extra = []
+    if args.only == "copy":
+        # calibration for the HBM traffic counters: device-to-device copies of
+        # exactly 256 MiB (read 256 MiB, write 256 MiB per copy)
+        nbytes = 256 << 20
+        src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+        capi.call("bs_memset", src.ptr, 1, nbytes, None)
+        fn = lambda i: L.bs_memcpy_d2d(dst.ptr, src.ptr, nbytes, stream.handle)  # noqa
+        ms = timed_passes(lambda i: capi.check(fn(i), "copy"), 1, args.steps, args.warmup,
+                          stream.handle)
+        print(json.dumps({"kernel": "copy", "bytes": nbytes, "ms_per_step": ms,
+                          "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
+    if args.only == "price":
+        fn = lambda i: L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle)  # noqa
+    elif args.only == "fast":
+        fn = lambda i: L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle)  # noqa
+    elif args.only == "greeks":
+        extra = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
+        fn = lambda i: L.bs_price_greeks_f32(*dbooks[i].ptrs(),  # noqa
+                                             *[o.ptr for o in extra[i]], n, stream.handle)
+    else:
+        h64 = host.astype(np.float64)
+        d64 = [capi.DeviceBook(h64, dtype=np.float64) for _ in range(2)]
+        o64 = [capi.alloc_out(n, np.float64) for _ in range(2)]
+        extra = [d64, o64]
+        nb = 2
+        fn = lambda i: L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, stream.handle)  # noqa
+    ms = timed_passes(lambda i: capi.check(fn(i), args.only), nb, args.steps, args.warmup,
+                      stream.handle)
+    print(json.dumps({"kernel": args.only, "options": n, "ms_per_step": ms}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +121,8 @@ def main() -> int:
     ap.add_argument("--cpu-passes", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy"],
+                    help="time one kernel only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -107,6 +146,8 @@ def main() -> int:
     def launch_price(i):
         capi.check(L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle), "price")
 
+    if args.only:
+        return only_one(args, L, host, dbooks, outs, stream)
     log("bench: price map")
     ms = timed_passes(launch_price, nb, args.steps, args.warmup, stream.handle)
     value = n / (ms * 1e-3)

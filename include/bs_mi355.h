@@ -62,8 +62,9 @@ enum {
     BS_CFG_CHECK_POINTERS = 1, /* 1 (default): validate device pointers      */
     BS_CFG_MAX_BLOCKS = 2,     /* grid cap for streaming kernels, 0 = auto    */
     BS_CFG_BLOCK_THREADS = 3,  /* read-only: threads per block (256)          */
-    BS_CFG_UNROLL = 4,         /* 4-option groups per thread per step (1,2,4)*/
-    BS_CFG_NONTEMPORAL = 5     /* 1 (default): non-temporal loads and stores  */
+    BS_CFG_VECTOR_WIDTH = 4,   /* options per work-item: 0 (default), 2, 4   */
+    BS_CFG_NONTEMPORAL = 5,    /* 1 (default): non-temporal loads and stores  */
+    BS_CFG_PIPELINE = 6        /* -1 (default), 0, 1: software pipelining     */
 };
 int bs_config_set(int key, int64_t value);
 int bs_config_get(int key, int64_t *value);
@@ -149,6 +150,7 @@ int bs_malloc(void **ptr, size_t bytes);
 int bs_free(void *ptr);
 int bs_memcpy_h2d(void *dst, const void *src, size_t bytes, bs_stream_t stream);
 int bs_memcpy_d2h(void *dst, const void *src, size_t bytes, bs_stream_t stream);
+int bs_memcpy_d2d(void *dst, const void *src, size_t bytes, bs_stream_t stream);
 int bs_memset(void *dst, int value, size_t bytes, bs_stream_t stream);
 int bs_stream_create(bs_stream_t *stream);
 int bs_stream_destroy(bs_stream_t stream);

@@ -27,7 +27,8 @@ STATUS = {
     -7: "BS_ERR_TIMEOUT", -8: "BS_ERR_BUSY", -9: "BS_ERR_NODEVICE",
     -10: "BS_ERR_FAILED", -11: "BS_ERR_SHUTDOWN",
 }
-CFG_CHECK_POINTERS, CFG_MAX_BLOCKS, CFG_BLOCK_THREADS, CFG_UNROLL, CFG_NONTEMPORAL = 1, 2, 3, 4, 5
+CFG_CHECK_POINTERS, CFG_MAX_BLOCKS, CFG_BLOCK_THREADS, CFG_VECTOR_WIDTH, CFG_NONTEMPORAL = 1, 2, 3, 4, 5
+CFG_PIPELINE = 6
 REQ_PRICE, REQ_GREEKS = 1, 2
 TICKET_QUEUED, TICKET_RUNNING, TICKET_DONE, TICKET_FAILED, TICKET_CANCELLED = range(5)
 
@@ -83,6 +84,7 @@ _SIGS = {
     "bs_free": ([_vp], _i32),
     "bs_memcpy_h2d": ([_vp, _vp, _sz, _vp], _i32),
     "bs_memcpy_d2h": ([_vp, _vp, _sz, _vp], _i32),
+    "bs_memcpy_d2d": ([_vp, _vp, _sz, _vp], _i32),
     "bs_memset": ([_vp, _i32, _sz, _vp], _i32),
     "bs_stream_create": ([ctypes.POINTER(_vp)], _i32),
     "bs_stream_destroy": ([_vp], _i32),

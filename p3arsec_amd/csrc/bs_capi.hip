@@ -135,12 +135,15 @@ int bs_config_set(int key, int64_t value) {
             if (value < 0) return set_error(BS_ERR_INVALID, "max_blocks < 0");
             config().max_blocks = value;
             return BS_OK;
-        case BS_CFG_UNROLL:
-            if (value != 1 && value != 2 && value != 4)
-                return set_error(BS_ERR_INVALID, "unroll must be 1, 2 or 4");
-            config().unroll = (int)value;
+        case BS_CFG_VECTOR_WIDTH:
+            if (value != 0 && value != 2 && value != 4)
+                return set_error(BS_ERR_INVALID, "vector width must be 0, 2 or 4");
+            config().vector_width = (int)value;
             return BS_OK;
         case BS_CFG_NONTEMPORAL: config().nontemporal = value != 0; return BS_OK;
+        case BS_CFG_PIPELINE:
+            config().pipeline = value < 0 ? -1 : (value != 0);
+            return BS_OK;
         default: return set_error(BS_ERR_INVALID, "unknown or read-only config key");
     }
 }
@@ -151,8 +154,9 @@ int bs_config_get(int key, int64_t* value) {
         case BS_CFG_CHECK_POINTERS: *value = config().check_pointers; return BS_OK;
         case BS_CFG_MAX_BLOCKS: *value = config().max_blocks; return BS_OK;
         case BS_CFG_BLOCK_THREADS: *value = kThreads; return BS_OK;
-        case BS_CFG_UNROLL: *value = config().unroll; return BS_OK;
+        case BS_CFG_VECTOR_WIDTH: *value = config().vector_width; return BS_OK;
         case BS_CFG_NONTEMPORAL: *value = config().nontemporal; return BS_OK;
+        case BS_CFG_PIPELINE: *value = config().pipeline; return BS_OK;
         default: return set_error(BS_ERR_INVALID, "unknown config key");
     }
 }
@@ -391,6 +395,10 @@ int bs_memcpy_h2d(void* dst, const void* src, size_t bytes, bs_stream_t stream) 
 int bs_memcpy_d2h(void* dst, const void* src, size_t bytes, bs_stream_t stream) {
     return done(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)),
                 "hipMemcpyAsync D2H");
+}
+int bs_memcpy_d2d(void* dst, const void* src, size_t bytes, bs_stream_t stream) {
+    return done(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)),
+                "hipMemcpyAsync D2D");
 }
 int bs_memset(void* dst, int value, size_t bytes, bs_stream_t stream) {
     return done(hipMemsetAsync(dst, value, bytes, as_stream(stream)), "hipMemsetAsync");

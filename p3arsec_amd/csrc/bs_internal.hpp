@@ -14,8 +14,9 @@ constexpr int kThreads = 256;  // threads per block: 4 wave64s
 struct Config {
     int check_pointers = 1;
     int64_t max_blocks = 0;  // 0: one 4-option group per thread, no grid cap
-    int unroll = 1;
+    int vector_width = 0;  // 0: each kernel's measured default (2 or 4)
     int nontemporal = 1;
+    int pipeline = -1;     // -1: each kernel's measured default
 };
 Config& config();
 

@@ -1,23 +1,31 @@
 // bs_kernels.hip -- gfx950 streaming kernels of the Black-Scholes map.
 //
 // Layout in HBM: structure of arrays, one array per field (s, k, r, v, t as
-// float or double, the option type as one byte).  A work-item prices a group
-// of 4 consecutive options: one 128-bit load per float array (two for
-// double), one 32-bit load for the 4 type bytes and one 128-bit store, so a
-// wave64 moves 1 KiB per array per instruction, fully coalesced.  Groups are
-// walked grid-stride (optionally U groups per step to keep more loads in
-// flight).  Loads and stores are non-temporal: every byte is touched once.
+// float or double, the option type as one byte).  A work-item handles a group
+// of VW consecutive options: one VW*4-byte load per float array, one VW-byte
+// load of the type bytes and one store per output array, so every wave64
+// instruction moves a contiguous 64*VW*4-byte span (fully coalesced).  VW is 4
+// (128-bit accesses) or 2 (64-bit): on MI355X the 2-wide form is the faster
+// one for every FP32 stream here (10M options: Greeks 71 vs 75 us, price 43
+// vs 46 us).  Groups are walked grid-stride, optionally software-pipelined
+// (the next group's loads issued before the current group's math).
+// Loads and stores are non-temporal: every byte is touched once.
 //
 // The pricing arithmetic lives in bs_math.hpp; this file only moves data.
+#include <algorithm>
+
 #include "bs_internal.hpp"
 #include "bs_math.hpp"
 
 namespace bs {
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef double d2 __attribute__((ext_vector_type(2)));
-typedef int i4 __attribute__((ext_vector_type(4)));
+template <class T, int N>
+using vec = T __attribute__((ext_vector_type(N)));
+
+// VW type bytes as one integer: uint16_t for 2 options, uint32_t for 4.
+template <int VW>
+using tword = typename std::conditional<VW == 4, uint32_t, uint16_t>::type;
 
 template <bool NT, class T>
 __device__ __forceinline__ T ld(const T* p) {
@@ -35,140 +43,159 @@ __device__ __forceinline__ bool put_of(uint32_t word, int j) {
 }
 
 // ---- per-group operations --------------------------------------------------
-// Each op: Group (registers of one 4-option group), load<NT>(g), finish<NT>(g,
-// x) (compute + store) and scalar(i) for options outside whole groups.
+// Each op: Group (registers of one VW-option group), load<NT>(g),
+// finish<NT>(g, x) (compute + store), scalar(i) for options outside whole
+// groups, kWaves (the occupancy the register allocator is held to: 8 waves
+// per SIMD = 64 VGPRs), kBlocks (default grid cap, 0 = one group per
+// work-item) and kPipe (default loop form); the defaults and each launcher's
+// preferred width are the fastest forms measured on MI355X for 10M options.
 
-template <class Math>
+template <class Math, int VW>
 struct PriceF32 {
+    static constexpr int kVW = VW;
+    static constexpr int kWaves = 8;
+    static constexpr int64_t kBlocks = 0;
+    static constexpr bool kPipe = true;
+    using F = vec<float, VW>;
     SoaIn<float> in;
     const uint8_t* type;
     float* out;
     struct Group {
-        f4 s, k, r, v, t;
-        uint32_t ty;
+        F s, k, r, v, t;
+        tword<VW> ty;
     };
     template <bool NT>
     __device__ __forceinline__ Group load(int64_t g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const f4*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const f4*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const f4*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const f4*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const f4*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const uint32_t*>(type) + g);
+        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
+        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
+        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
+        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
+        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
+        x.ty = ld<NT>(reinterpret_cast<const tword<VW>*>(type) + g);
         return x;
-    }
-    __device__ __forceinline__ static float one(float s, float k, float r, float v, float t,
-                                                bool put) {
-        return bs_price<Math>(s, k, r, v, t, put);
     }
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        f4 p;
+        F p;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) p[j] = one(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
-        st<NT>(reinterpret_cast<f4*>(out) + g, p);
+        for (int j = 0; j < VW; ++j)
+            p[j] = bs_price<Math>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
+        st<NT>(reinterpret_cast<F*>(out) + g, p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = one(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
+        out[i] = bs_price<Math>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
     }
 };
 
+template <int VW>
 struct PriceI32F32 {
+    static constexpr int kVW = VW;
+    static constexpr int kWaves = 7;
+    static constexpr int64_t kBlocks = 0;
+    static constexpr bool kPipe = true;
+    using F = vec<float, VW>;
+    using I = vec<int, VW>;
     SoaIn<float> in;
     const int32_t* type;
     float* out;
     struct Group {
-        f4 s, k, r, v, t;
-        i4 ty;
+        F s, k, r, v, t;
+        I ty;
     };
     template <bool NT>
     __device__ __forceinline__ Group load(int64_t g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const f4*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const f4*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const f4*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const f4*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const f4*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const i4*>(type) + g);
+        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
+        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
+        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
+        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
+        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
+        x.ty = ld<NT>(reinterpret_cast<const I*>(type) + g);
         return x;
     }
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        f4 p;
+        F p;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < VW; ++j)
             p[j] = bs_price<Ref32>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], x.ty[j] != 0);
-        st<NT>(reinterpret_cast<f4*>(out) + g, p);
+        st<NT>(reinterpret_cast<F*>(out) + g, p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = bs_price<Ref32>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
     }
 };
 
+// fptype = double: a group is VW = 2 options (one 128-bit load per array).
+template <int VW>
 struct PriceF64 {
+    static constexpr int kVW = 2;
+    static constexpr int kWaves = 5;
+    static constexpr int64_t kBlocks = 0;
+    static constexpr bool kPipe = true;
+    using D = vec<double, 2>;
     SoaIn<double> in;
     const uint8_t* type;
     double* out;
     struct Group {
-        d2 s[2], k[2], r[2], v[2], t[2];
-        uint32_t ty;
+        D s, k, r, v, t;
+        uint16_t ty;
     };
     template <bool NT>
     __device__ __forceinline__ Group load(int64_t g) const {
         Group x;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            x.s[h] = ld<NT>(reinterpret_cast<const d2*>(in.s) + 2 * g + h);
-            x.k[h] = ld<NT>(reinterpret_cast<const d2*>(in.k) + 2 * g + h);
-            x.r[h] = ld<NT>(reinterpret_cast<const d2*>(in.r) + 2 * g + h);
-            x.v[h] = ld<NT>(reinterpret_cast<const d2*>(in.v) + 2 * g + h);
-            x.t[h] = ld<NT>(reinterpret_cast<const d2*>(in.t) + 2 * g + h);
-        }
-        x.ty = ld<NT>(reinterpret_cast<const uint32_t*>(type) + g);
+        x.s = ld<NT>(reinterpret_cast<const D*>(in.s) + g);
+        x.k = ld<NT>(reinterpret_cast<const D*>(in.k) + g);
+        x.r = ld<NT>(reinterpret_cast<const D*>(in.r) + g);
+        x.v = ld<NT>(reinterpret_cast<const D*>(in.v) + g);
+        x.t = ld<NT>(reinterpret_cast<const D*>(in.t) + g);
+        x.ty = ld<NT>(reinterpret_cast<const uint16_t*>(type) + g);
         return x;
     }
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
+        D p;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            d2 p;
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                p[j] = bs_price<Exact64>(x.s[h][j], x.k[h][j], x.r[h][j], x.v[h][j],
-                                         x.t[h][j], put_of(x.ty, 2 * h + j));
-            st<NT>(reinterpret_cast<d2*>(out) + 2 * g + h, p);
-        }
+        for (int j = 0; j < 2; ++j)
+            p[j] = bs_price<Exact64>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
+        st<NT>(reinterpret_cast<D*>(out) + g, p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = bs_price<Exact64>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
     }
 };
 
+template <int VW>
 struct GreeksF32 {
+    static constexpr int kVW = VW;
+    static constexpr int kWaves = VW == 4 ? 5 : 8;
+    static constexpr int64_t kBlocks = 0;
+    static constexpr bool kPipe = false;
+    using F = vec<float, VW>;
     SoaIn<float> in;
     const uint8_t* type;
     GreeksOut out;
     struct Group {
-        f4 s, k, r, v, t;
-        uint32_t ty;
+        F s, k, r, v, t;
+        tword<VW> ty;
     };
     template <bool NT>
     __device__ __forceinline__ Group load(int64_t g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const f4*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const f4*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const f4*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const f4*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const f4*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const uint32_t*>(type) + g);
+        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
+        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
+        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
+        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
+        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
+        x.ty = ld<NT>(reinterpret_cast<const tword<VW>*>(type) + g);
         return x;
     }
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        f4 p, d, ve, rh, th;
+        F p, d, ve, rh, th;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < VW; ++j) {
             const Greeks32 o = bs_greeks<Ref32>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j],
                                                 put_of(x.ty, j));
             p[j] = o.price;
@@ -177,11 +204,11 @@ struct GreeksF32 {
             rh[j] = o.rho;
             th[j] = o.theta;
         }
-        st<NT>(reinterpret_cast<f4*>(out.price) + g, p);
-        st<NT>(reinterpret_cast<f4*>(out.delta) + g, d);
-        st<NT>(reinterpret_cast<f4*>(out.vega) + g, ve);
-        st<NT>(reinterpret_cast<f4*>(out.rho) + g, rh);
-        st<NT>(reinterpret_cast<f4*>(out.theta) + g, th);
+        st<NT>(reinterpret_cast<F*>(out.price) + g, p);
+        st<NT>(reinterpret_cast<F*>(out.delta) + g, d);
+        st<NT>(reinterpret_cast<F*>(out.vega) + g, ve);
+        st<NT>(reinterpret_cast<F*>(out.rho) + g, rh);
+        st<NT>(reinterpret_cast<F*>(out.theta) + g, th);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         const Greeks32 o = bs_greeks<Ref32>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i],
@@ -194,73 +221,133 @@ struct GreeksF32 {
     }
 };
 
+template <int VW>
 struct CndfF32 {
+    static constexpr int kVW = VW;
+    static constexpr int kWaves = 8;
+    static constexpr int64_t kBlocks = 0;
+    static constexpr bool kPipe = false;
+    using F = vec<float, VW>;
     const float* x;
     float* out;
     struct Group {
-        f4 x;
+        F x;
     };
     template <bool NT>
     __device__ __forceinline__ Group load(int64_t g) const {
-        return Group{ld<NT>(reinterpret_cast<const f4*>(x) + g)};
+        return Group{ld<NT>(reinterpret_cast<const F*>(x) + g)};
     }
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
-        f4 y;
+        F y;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) y[j] = Ref32::cndf<false>(a.x[j], nullptr);
-        st<NT>(reinterpret_cast<f4*>(out) + g, y);
+        for (int j = 0; j < VW; ++j) y[j] = Ref32::cndf<false>(a.x[j], nullptr);
+        st<NT>(reinterpret_cast<F*>(out) + g, y);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = Ref32::cndf<false>(x[i], nullptr);
     }
 };
 
+template <int VW>
 struct CndfF64 {
+    static constexpr int kVW = 2;
+    static constexpr int kWaves = 8;
+    static constexpr int64_t kBlocks = 0;
+    static constexpr bool kPipe = false;
+    using D = vec<double, 2>;
     const double* x;
     double* out;
     struct Group {
-        d2 x[2];
+        D x;
     };
     template <bool NT>
     __device__ __forceinline__ Group load(int64_t g) const {
-        Group a;
-        a.x[0] = ld<NT>(reinterpret_cast<const d2*>(x) + 2 * g);
-        a.x[1] = ld<NT>(reinterpret_cast<const d2*>(x) + 2 * g + 1);
-        return a;
+        return Group{ld<NT>(reinterpret_cast<const D*>(x) + g)};
     }
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            d2 y;
-            y[0] = Exact64::cndf<false>(a.x[h][0], nullptr);
-            y[1] = Exact64::cndf<false>(a.x[h][1], nullptr);
-            st<NT>(reinterpret_cast<d2*>(out) + 2 * g + h, y);
-        }
+        D y;
+        y[0] = Exact64::cndf<false>(a.x[0], nullptr);
+        y[1] = Exact64::cndf<false>(a.x[1], nullptr);
+        st<NT>(reinterpret_cast<D*>(out) + g, y);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = Exact64::cndf<false>(x[i], nullptr);
     }
 };
 
+// ---- argument packs and their op constructors ----------------------------------
+struct PriceArgs {
+    SoaIn<float> in;
+    const uint8_t* type;
+    float* out;
+};
+struct PriceI32Args {
+    SoaIn<float> in;
+    const int32_t* type;
+    float* out;
+};
+struct PriceF64Args {
+    SoaIn<double> in;
+    const uint8_t* type;
+    double* out;
+};
+struct GreeksArgs {
+    SoaIn<float> in;
+    const uint8_t* type;
+    GreeksOut out;
+};
+template <class T>
+struct CndfArgs {
+    const T* x;
+    T* out;
+};
+
+template <class Op>
+Op make_op(const PriceArgs& a) { return Op{a.in, a.type, a.out}; }
+template <class Op>
+Op make_op(const PriceI32Args& a) { return Op{a.in, a.type, a.out}; }
+template <class Op>
+Op make_op(const PriceF64Args& a) { return Op{a.in, a.type, a.out}; }
+template <class Op>
+Op make_op(const GreeksArgs& a) { return Op{a.in, a.type, a.out}; }
+template <class Op>
+Op make_op(const CndfArgs<float>& a) { return Op{a.x, a.out}; }
+template <class Op>
+Op make_op(const CndfArgs<double>& a) { return Op{a.x, a.out}; }
+
+
 // ---- generic streaming kernels ----------------------------------------------
-// Grid-stride over `ngroups` 4-option groups, U groups per step; options
-// [tail_begin, n) (fewer than 4) are priced one per thread by block 0.
-template <class Op, int U, bool NT>
-__global__ __launch_bounds__(kThreads) void k_stream(Op op, int64_t ngroups,
-                                                     int64_t tail_begin, int64_t n) {
+// Grid-stride over `ngroups` groups of VW options; the options [tail_begin, n)
+// that do not fill a group are priced one per work-item by block 0.  PIPE
+// issues the loads of a work-item's next group before pricing the current one
+// (register double buffering), so a wave overlaps its own HBM latency with its
+// arithmetic; it holds one more group of registers, so its occupancy target
+// is two waves lower.
+template <class Op, bool PIPE>
+constexpr int waves_for() {
+    return PIPE ? (Op::kWaves > 5 ? Op::kWaves - 2 : 3) : Op::kWaves;
+}
+
+template <class Op, bool NT, bool PIPE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, PIPE>())))
+void k_stream(Op op, int64_t ngroups, int64_t tail_begin, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * kThreads;
     const int64_t gid = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     int64_t g = gid;
-    for (; g + (U - 1) * stride < ngroups; g += U * stride) {
-        typename Op::Group x[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) x[u] = op.template load<NT>(g + u * stride);
-#pragma unroll
-        for (int u = 0; u < U; ++u) op.template finish<NT>(g + u * stride, x[u]);
-    }
-    if constexpr (U > 1) {
+    if constexpr (PIPE) {
+        if (g < ngroups) {
+            typename Op::Group cur = op.template load<NT>(g);
+            for (int64_t gn = g + stride; gn < ngroups; gn += stride) {
+                const typename Op::Group nxt = op.template load<NT>(gn);
+                op.template finish<NT>(g, cur);
+                cur = nxt;
+                g = gn;
+            }
+            op.template finish<NT>(g, cur);
+        }
+    } else {
         for (; g < ngroups; g += stride) {
             const typename Op::Group x = op.template load<NT>(g);
             op.template finish<NT>(g, x);
@@ -269,7 +356,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(Op op, int64_t ngroups,
     if (gid < n - tail_begin) op.scalar(tail_begin + gid);
 }
 
-// One option per thread, any alignment.
+// One option per work-item, any alignment.
 template <class Op>
 __global__ __launch_bounds__(kThreads) void k_scalar(Op op, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -281,37 +368,50 @@ inline bool aligned(const void* p, uintptr_t a) {
     return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
 }
 
-inline unsigned grid_for(int64_t work_items) {
+inline unsigned grid_for(int64_t work_items, int64_t cap) {
     int64_t blocks = (work_items + kThreads - 1) / kThreads;
-    const int64_t cap = config().max_blocks;
     if (cap > 0 && blocks > cap) blocks = cap;
     if (blocks > 0x7fffffff) blocks = 0x7fffffff;
     return (unsigned)(blocks < 1 ? 1 : blocks);
 }
 
-template <class Op, int U, bool NT>
-hipError_t launch_u(const Op& op, int64_t n, hipStream_t st) {
-    const int64_t ngroups = n / 4;
-    const int64_t per_thread_groups = U;
-    unsigned grid = grid_for((ngroups + per_thread_groups - 1) / per_thread_groups);
-    hipLaunchKernelGGL((k_stream<Op, U, NT>), dim3(grid), dim3(kThreads), 0, st, op,
-                       ngroups, ngroups * 4, n);
+template <class Op, bool NT, bool PIPE>
+hipError_t launch_form(const Op& op, int64_t n, hipStream_t st) {
+    constexpr int VW = Op::kVW;
+    const int64_t ngroups = n / VW;
+    const int64_t cap = config().max_blocks > 0 ? config().max_blocks : Op::kBlocks;
+    const unsigned grid = grid_for(std::max<int64_t>(ngroups, 1), cap);
+    hipLaunchKernelGGL((k_stream<Op, NT, PIPE>), dim3(grid), dim3(kThreads), 0, st, op, ngroups,
+                       ngroups * VW, n);
     return hipGetLastError();
 }
 
 template <class Op>
-hipError_t launch_stream(const Op& op, int64_t n, bool vector_ok, hipStream_t st) {
+hipError_t launch_op(const Op& op, int64_t n, hipStream_t st) {
+    const bool nt = config().nontemporal != 0;
+    const bool pipe = config().pipeline < 0 ? Op::kPipe : config().pipeline != 0;
+    if (pipe)
+        return nt ? launch_form<Op, true, true>(op, n, st) : launch_form<Op, false, true>(op, n, st);
+    return nt ? launch_form<Op, true, false>(op, n, st) : launch_form<Op, false, false>(op, n, st);
+}
+
+// Instantiates OP<VW> for the configured (or the op's preferred) width;
+// `align` gives the byte alignment every pointer needs for width VW, and a
+// misaligned call takes the one-option-per-work-item kernel.
+template <template <int> class OP, class Args, class AlignFn>
+hipError_t launch_stream(const Args& a, int64_t n, int preferred_vw, AlignFn aligned_for,
+                         hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (!vector_ok) {
-        hipLaunchKernelGGL((k_scalar<Op>), dim3(grid_for(n)), dim3(kThreads), 0, st, op, n);
+    int vw = config().vector_width > 0 ? config().vector_width : preferred_vw;
+    if (vw != 2 && vw != 4) vw = 4;
+    if (vw == 4 && !aligned_for(4)) vw = 2;
+    if (!aligned_for(vw)) {
+        const OP<2> op = make_op<OP<2>>(a);
+        hipLaunchKernelGGL((k_scalar<OP<2>>), dim3(grid_for(n, 0)), dim3(kThreads), 0, st, op, n);
         return hipGetLastError();
     }
-    const bool nt = config().nontemporal != 0;
-    switch (config().unroll) {
-        case 2: return nt ? launch_u<Op, 2, true>(op, n, st) : launch_u<Op, 2, false>(op, n, st);
-        case 4: return nt ? launch_u<Op, 4, true>(op, n, st) : launch_u<Op, 4, false>(op, n, st);
-        default: return nt ? launch_u<Op, 1, true>(op, n, st) : launch_u<Op, 1, false>(op, n, st);
-    }
+    if (vw == 4) return launch_op(make_op<OP<4>>(a), n, st);
+    return launch_op(make_op<OP<2>>(a), n, st);
 }
 
 template <class T>
@@ -366,45 +466,67 @@ __global__ void k_cndf_scalar(float x, float* out) { out[0] = Ref32::cndf<false>
 
 }  // namespace
 
+namespace {
+template <int VW>
+using PriceRef = PriceF32<Ref32, VW>;
+template <int VW>
+using PriceFast = PriceF32<Fast32, VW>;
+
+// Alignment test of a price-shaped call for width vw: floats need 4*vw bytes,
+// the uint8 type array vw bytes (int32 types: 4*vw).
+template <class In, class Ty, class Out>
+auto price_align(const In& in, const Ty* type, const Out* out, size_t ty_elem) {
+    return [=](int vw) {
+        const uintptr_t fa = sizeof(typename std::remove_pointer<decltype(in.s)>::type) * vw;
+        return soa_aligned(in, fa) && aligned(out, fa) && aligned(type, ty_elem * vw);
+    };
+}
+}  // namespace
+
 // ---- launchers ---------------------------------------------------------------
 hipError_t launch_price_f32(SoaIn<float> in, const uint8_t* type, float* out, int64_t n,
                             hipStream_t st) {
-    const bool vec = soa_aligned(in, 16) && aligned(out, 16) && aligned(type, 4);
-    return launch_stream(PriceF32<Ref32>{in, type, out}, n, vec, st);
+    return launch_stream<PriceRef>(PriceArgs{in, type, out}, n, 2, price_align(in, type, out, 1),
+                                   st);
 }
 
 hipError_t launch_price_fast_f32(SoaIn<float> in, const uint8_t* type, float* out,
                                  int64_t n, hipStream_t st) {
-    const bool vec = soa_aligned(in, 16) && aligned(out, 16) && aligned(type, 4);
-    return launch_stream(PriceF32<Fast32>{in, type, out}, n, vec, st);
+    return launch_stream<PriceFast>(PriceArgs{in, type, out}, n, 2, price_align(in, type, out, 1),
+                                    st);
 }
 
 hipError_t launch_price_i32type_f32(SoaIn<float> in, const int32_t* type, float* out,
                                     int64_t n, hipStream_t st) {
-    const bool vec = soa_aligned(in, 16) && aligned(out, 16) && aligned(type, 16);
-    return launch_stream(PriceI32F32{in, type, out}, n, vec, st);
+    return launch_stream<PriceI32F32>(PriceI32Args{in, type, out}, n, 2,
+                                      price_align(in, type, out, 4), st);
 }
 
 hipError_t launch_price_f64(SoaIn<double> in, const uint8_t* type, double* out, int64_t n,
                             hipStream_t st) {
-    const bool vec = soa_aligned(in, 16) && aligned(out, 16) && aligned(type, 4);
-    return launch_stream(PriceF64{in, type, out}, n, vec, st);
+    auto al = [=](int) { return soa_aligned(in, 16) && aligned(out, 16) && aligned(type, 2); };
+    return launch_stream<PriceF64>(PriceF64Args{in, type, out}, n, 2, al, st);
 }
 
 hipError_t launch_greeks_f32(SoaIn<float> in, const uint8_t* type, GreeksOut out, int64_t n,
                              hipStream_t st) {
-    const bool vec = soa_aligned(in, 16) && aligned(type, 4) && aligned(out.price, 16) &&
-                     aligned(out.delta, 16) && aligned(out.vega, 16) &&
-                     aligned(out.rho, 16) && aligned(out.theta, 16);
-    return launch_stream(GreeksF32{in, type, out}, n, vec, st);
+    auto al = [=](int vw) {
+        const uintptr_t fa = 4 * vw;
+        return soa_aligned(in, fa) && aligned(type, vw) && aligned(out.price, fa) &&
+               aligned(out.delta, fa) && aligned(out.vega, fa) && aligned(out.rho, fa) &&
+               aligned(out.theta, fa);
+    };
+    return launch_stream<GreeksF32>(GreeksArgs{in, type, out}, n, 2, al, st);
 }
 
 hipError_t launch_cndf_f32(const float* x, float* out, int64_t n, hipStream_t st) {
-    return launch_stream(CndfF32{x, out}, n, aligned(x, 16) && aligned(out, 16), st);
+    auto al = [=](int vw) { return aligned(x, 4 * vw) && aligned(out, 4 * vw); };
+    return launch_stream<CndfF32>(CndfArgs<float>{x, out}, n, 4, al, st);
 }
 
 hipError_t launch_cndf_f64(const double* x, double* out, int64_t n, hipStream_t st) {
-    return launch_stream(CndfF64{x, out}, n, aligned(x, 16) && aligned(out, 16), st);
+    auto al = [=](int) { return aligned(x, 16) && aligned(out, 16); };
+    return launch_stream<CndfF64>(CndfArgs<double>{x, out}, n, 2, al, st);
 }
 
 hipError_t launch_repack_aos(const void* aos, int64_t n, float* s, float* k, float* r,

@@ -98,19 +98,20 @@ __device__ __forceinline__ float mul_dc(float a, SplitConst c) {
     return __builtin_fmaf(a, c.hi, a * c.lo);
 }
 
-// exp(x) from the hardware 2^x with the argument carried in two floats:
-// x log2(e) = th + tl exactly enough (tl from an FMA and the low half of
-// log2(e)), then 2^(th + tl) = 2^th (1 + tl ln2).  Measured on gfx950: 87% of
-// results equal the correctly rounded float, the rest are one ulp away.  The
-// argument is clamped to [-150, 89] (results 0 and +inf outside, as expf).
-constexpr float kLog2eLo = 1.925963033500011e-08f;   // log2(e) - (float)log2(e)
+// exp(x) = 2^th e^r with th = round(x log2(e)) and r = x - th ln2 carried
+// exactly (Cody-Waite: ln2 split in two floats, each product in an FMA), then
+// e^r = 1 + r to float precision since |r| <= 2^-23 |x|.  Measured on gfx950:
+// 87% of results equal the correctly rounded float, the rest are one ulp
+// away.  The argument is clamped to [-150, 89] (results 0 and +inf outside,
+// as expf gives).
+constexpr float kLn2Hi = 0.693145751953125f;        // 11 significant bits
+constexpr float kLn2Lo = 1.428606820309417232e-06f;  // ln2 - kLn2Hi
 __device__ __forceinline__ float exp_hw(float x) {
     x = __builtin_amdgcn_fmed3f(x, -150.0f, 89.0f);
     const float th = x * kLog2e;
-    float tl = __builtin_fmaf(x, kLog2e, -th);
-    tl = __builtin_fmaf(x, kLog2eLo, tl);
+    const float r = __builtin_fmaf(-th, kLn2Lo, __builtin_fmaf(-th, kLn2Hi, x));
     const float e = __builtin_amdgcn_exp2f(th);
-    return __builtin_fmaf(e, tl * kLn2f, e);
+    return __builtin_fmaf(e, r, e);
 }
 
 // log(x) = log2(x) ln2 from the hardware v_log_f32 (70% correctly rounded,
@@ -144,7 +145,7 @@ struct Ref32 {
                              b2 = split(kB2), b3 = split(kB3), b4 = split(kB4),
                              b5 = split(kB5);
         const bool neg = x < 0.0f;
-        const float ax = neg ? -x : x;
+        const float ax = __builtin_fabsf(x);   // |-0| = +0 gives the same bits below
         float n = exp(-0.5f * ax * ax);
         n = mul_dc(n, c0);
         float k = mul_dc(ax, cp);

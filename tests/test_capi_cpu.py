@@ -36,11 +36,11 @@ def test_status_strings():
 
 def test_config_roundtrip_and_validation():
     L = capi.load()
-    old = capi.get_config(capi.CFG_UNROLL)
-    capi.set_config(capi.CFG_UNROLL, 2)
-    assert capi.get_config(capi.CFG_UNROLL) == 2
-    capi.set_config(capi.CFG_UNROLL, old)
-    assert L.bs_config_set(capi.CFG_UNROLL, 3) == -1
+    old = capi.get_config(capi.CFG_VECTOR_WIDTH)
+    capi.set_config(capi.CFG_VECTOR_WIDTH, 2)
+    assert capi.get_config(capi.CFG_VECTOR_WIDTH) == 2
+    capi.set_config(capi.CFG_VECTOR_WIDTH, old)
+    assert L.bs_config_set(capi.CFG_VECTOR_WIDTH, 3) == -1
     assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 64) == -1   # read-only
     assert L.bs_config_set(999, 1) == -1
     assert capi.get_config(capi.CFG_BLOCK_THREADS) == 256

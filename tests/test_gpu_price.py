@@ -32,6 +32,13 @@ def run(capi, book, mode="ref", **kw):
     return got
 
 
+def reset_config(capi):
+    capi.set_config(capi.CFG_MAX_BLOCKS, 0)
+    capi.set_config(capi.CFG_VECTOR_WIDTH, 0)
+    capi.set_config(capi.CFG_NONTEMPORAL, 1)
+    capi.set_config(capi.CFG_PIPELINE, -1)
+
+
 def err_stats(got, ref):
     e = np.abs(got.astype(np.float64) - ref.astype(np.float64))
     return float(e.max()) if e.size else 0.0, float(e.mean()) if e.size else 0.0
@@ -81,17 +88,14 @@ def test_launch_shape_and_alignment_do_not_change_bits(gpu):
     variants = [run(gpu, book, pad=1), run(gpu, book, out_pad=1),   # scalar path
                 run(gpu, book, "i32")]
     try:
-        for key, val in ((gpu.CFG_MAX_BLOCKS, 7), (gpu.CFG_UNROLL, 2), (gpu.CFG_UNROLL, 4),
-                         (gpu.CFG_NONTEMPORAL, 0)):
+        for key, val in ((gpu.CFG_MAX_BLOCKS, 7), (gpu.CFG_VECTOR_WIDTH, 2),
+                         (gpu.CFG_VECTOR_WIDTH, 4), (gpu.CFG_NONTEMPORAL, 0),
+                         (gpu.CFG_PIPELINE, 1), (gpu.CFG_PIPELINE, 0)):
             gpu.set_config(key, val)
             variants.append(run(gpu, book))
-            gpu.set_config(gpu.CFG_MAX_BLOCKS, 0)
-            gpu.set_config(gpu.CFG_UNROLL, 1)
-            gpu.set_config(gpu.CFG_NONTEMPORAL, 1)
+            reset_config(gpu)
     finally:
-        gpu.set_config(gpu.CFG_MAX_BLOCKS, 0)
-        gpu.set_config(gpu.CFG_UNROLL, 1)
-        gpu.set_config(gpu.CFG_NONTEMPORAL, 1)
+        reset_config(gpu)
     for v in variants:
         assert np.array_equal(v.view(np.uint32), base.view(np.uint32))
 
