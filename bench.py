@@ -32,6 +32,7 @@ from p3arsec_amd import books, capi  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = ("options/sec & % HBM roofline (10M fp32); Greeks cost ratio; serve p50/p99 ms")
+ARGS = None
 
 
 def log(msg: str) -> None:
@@ -72,6 +73,31 @@ def cpu_baseline(book_host, passes: int) -> dict:
             "sample": f"{book_host.n} options x {passes} passes, static blocks, "
                       f"{used} pinned threads, oracle/bs_oracle.c (bit-exact with "
                       f"blackscholes.c), {sec:.2f} s"}
+
+
+def serve_bench(result: dict) -> dict:
+    """Runs the serving load generator (p3arsec_amd/bin/bs_serve_bench, a C++
+    client of the C ABI) as a child process: open-loop Poisson at 20K req/s
+    and a closed-loop saturation run; compares the saturated options/s with
+    the batch kernels' options/s for the same 70/30 price/Greeks mix."""
+    import subprocess
+    exe = os.path.join(ROOT, "p3arsec_amd", "bin", "bs_serve_bench")
+    secs = str(ARGS.serve_seconds)
+    out = {}
+    for tag, extra in (("poisson_20k", ["--rate", "20000"]),
+                       ("saturation", ["--saturate", "--inflight", "512"])):
+        p = subprocess.run([exe, "--duration-s", secs, "--budget-gb", "32"] + extra,
+                           capture_output=True, text=True, timeout=300)
+        if p.returncode != 0:
+            raise RuntimeError(f"bs_serve_bench {tag} failed: {p.stderr[-2000:]}")
+        out[tag] = json.loads(p.stdout.strip().splitlines()[-1])
+    price_rate = result["value"]
+    greeks_rate = result.get("greeks", {}).get("options_per_s")
+    if greeks_rate:
+        batch = 1.0 / (0.7 / price_rate + 0.3 / greeks_rate)
+        out["batch_mode_options_per_s_70_30"] = batch
+        out["saturation_frac_of_batch"] = out["saturation"]["options_per_s"] / batch
+    return out
 
 
 def only_one(args, L, host, dbooks, outs, stream) -> int:
@@ -121,9 +147,13 @@ def main() -> int:
     ap.add_argument("--cpu-passes", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy"],
                     help="time one kernel only (for profiler runs)")
     args = ap.parse_args()
+    global ARGS
+    ARGS = args
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the map needs no collectives; sharding is described in DESIGN.md)",
@@ -214,6 +244,10 @@ def main() -> int:
             b.free()
         for o in o64:
             o.free()
+
+    if not args.no_extra and not args.no_serve:
+        log("bench: serving loop")
+        result["serve"] = serve_bench(result)
 
     if not args.no_cpu:
         log("bench: cpu baseline")

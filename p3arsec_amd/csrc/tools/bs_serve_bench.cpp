@@ -1,0 +1,219 @@
+// bs_serve_bench -- load generator for the serving loop (libbs_mi355.so).
+//
+// Registers a catalog of resident books whose sizes are log-uniform on
+// [1, 2^20] options (seeded, the same option distributions as the batch
+// benchmark), then either
+//   * open loop: Poisson arrivals at --rate requests/s for --duration-s,
+//     each request a random book, BS_REQ_GREEKS with probability
+//     --greeks-frac, else BS_REQ_PRICE; or
+//   * --saturate: closed loop keeping --inflight requests outstanding, to
+//     measure the loop's throughput ceiling.
+// Latency = scheduled arrival -> completion observed by the dispatcher (the
+// request's outputs are resident in HBM and readable), both on
+// CLOCK_MONOTONIC.  Prints one JSON object.
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "bs_mi355.h"
+
+namespace {
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+void die(const char* what, int rc) {
+    fprintf(stderr, "%s failed: %d (%s) %s\n", what, rc, bs_status_string(rc), bs_last_error());
+    exit(1);
+}
+#define CHECK(call)                      \
+    do {                                 \
+        int rc_ = (call);                \
+        if (rc_ != BS_OK) die(#call, rc_); \
+    } while (0)
+
+struct Opts {
+    double rate = 20000.0, duration_s = 5.0, greeks_frac = 0.3, budget_gb = 32.0;
+    int books = 256, inflight = 256, use_graph = 1;
+    int64_t max_book = 1 << 20, small = 65536;
+    uint64_t seed = 20261207;
+    bool saturate = false;
+};
+
+struct Book {
+    int64_t id, n;
+};
+
+struct Req {
+    int64_t ticket, n, arrival;
+    int kind;
+};
+
+double pct(std::vector<double>& v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const size_t i = std::min(v.size() - 1, (size_t)std::ceil(p * (double)v.size()) - (p > 0 ? 1 : 0));
+    return v[i];
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Opts o;
+    for (int i = 1; i < argc; ++i) {
+        auto val = [&](const char* name) -> const char* {
+            if (strcmp(argv[i], name) == 0 && i + 1 < argc) return argv[++i];
+            return nullptr;
+        };
+        const char* v;
+        if ((v = val("--rate"))) o.rate = atof(v);
+        else if ((v = val("--duration-s"))) o.duration_s = atof(v);
+        else if ((v = val("--greeks-frac"))) o.greeks_frac = atof(v);
+        else if ((v = val("--budget-gb"))) o.budget_gb = atof(v);
+        else if ((v = val("--books"))) o.books = atoi(v);
+        else if ((v = val("--inflight"))) o.inflight = atoi(v);
+        else if ((v = val("--max-book"))) o.max_book = atoll(v);
+        else if ((v = val("--seed"))) o.seed = strtoull(v, nullptr, 10);
+        else if ((v = val("--use-graph"))) o.use_graph = atoi(v);
+        else if (strcmp(argv[i], "--saturate") == 0) o.saturate = true;
+        else {
+            fprintf(stderr, "unknown argument %s\n", argv[i]);
+            return 2;
+        }
+    }
+
+    bs_serve_opts so{};
+    so.use_graph = o.use_graph;
+    bs_ctx_t ctx = nullptr;
+    CHECK(bs_ctx_create(0, (size_t)(o.budget_gb * (double)(1ull << 30)), &so, &ctx));
+
+    // catalog: log-uniform sizes, options drawn as in the batch benchmark
+    std::mt19937_64 rng(o.seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<Book> books;
+    int64_t catalog_options = 0;
+    for (int b = 0; b < o.books; ++b) {
+        const int64_t n = std::max<int64_t>(
+            1, (int64_t)std::floor(std::exp(U(rng) * std::log((double)o.max_book + 1.0))));
+        std::vector<float> s(n), k(n), r(n), v(n), t(n);
+        std::vector<uint8_t> ty(n);
+        for (int64_t i = 0; i < n; ++i) {
+            s[i] = (float)(10.0 + 190.0 * U(rng));
+            k[i] = (float)(s[i] * (0.5 + U(rng)));
+            r[i] = (float)(0.01 + 0.09 * U(rng));
+            v[i] = (float)(0.05 + 0.60 * U(rng));
+            t[i] = (float)(0.05 + 1.95 * U(rng));
+            ty[i] = U(rng) < 0.5 ? 1 : 0;
+        }
+        bs_book_t id;
+        CHECK(bs_book_register(ctx, s.data(), k.data(), r.data(), v.data(), t.data(), ty.data(), n,
+                               &id));
+        books.push_back({id, n});
+        catalog_options += n;
+    }
+
+    std::mutex mu;
+    std::deque<Req> pending;
+    std::atomic<bool> done_submitting{false};
+    std::vector<double> lat_all, lat_small;
+    int64_t options_done = 0, requests_done = 0;
+    std::atomic<int64_t> outstanding{0};
+
+    std::thread completer([&] {
+        for (;;) {
+            Req q;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (pending.empty()) {
+                    if (done_submitting.load()) break;
+                    q.ticket = -1;
+                } else {
+                    q = pending.front();
+                    pending.pop_front();
+                }
+            }
+            if (q.ticket < 0) {
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+                continue;
+            }
+            CHECK(bs_wait(ctx, q.ticket, -1));
+            int64_t sub, fin;
+            CHECK(bs_ticket_times(ctx, q.ticket, &sub, &fin));
+            CHECK(bs_result_release(ctx, q.ticket));
+            outstanding.fetch_sub(1);
+            const double ms = (double)(fin - q.arrival) * 1e-6;
+            lat_all.push_back(ms);
+            if (q.n <= o.small) lat_small.push_back(ms);
+            options_done += q.n;
+            ++requests_done;
+        }
+    });
+
+    std::exponential_distribution<double> gap(o.rate);
+    std::uniform_int_distribution<int> pick(0, (int)books.size() - 1);
+    const int64_t t0 = now_ns();
+    const int64_t t_end = t0 + (int64_t)(o.duration_s * 1e9);
+    double next = (double)t0;
+    int64_t submitted = 0;
+    for (;;) {
+        int64_t arrival;
+        if (o.saturate) {
+            while (outstanding.load() >= o.inflight) std::this_thread::yield();
+            arrival = now_ns();
+            if (arrival >= t_end) break;
+        } else {
+            next += gap(rng) * 1e9;
+            arrival = (int64_t)next;
+            if (arrival >= t_end) break;
+            while (now_ns() < arrival) {
+            }
+        }
+        const Book& b = books[pick(rng)];
+        const int kind = U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE;
+        bs_ticket_t tk;
+        CHECK(bs_submit(ctx, b.id, kind, &tk));
+        outstanding.fetch_add(1);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            pending.push_back({tk, b.n, arrival, kind});
+        }
+        ++submitted;
+    }
+    done_submitting.store(true);
+    completer.join();
+    const double wall = (double)(now_ns() - t0) * 1e-9;
+    bs_serve_stats st;
+    CHECK(bs_ctx_stats(ctx, &st));
+    CHECK(bs_ctx_destroy(ctx));
+
+    printf("{\"mode\": \"%s\", \"rate_req_s\": %.1f, \"duration_s\": %.3f, \"requests\": %lld, "
+           "\"options\": %lld, \"options_per_s\": %.6e, \"p50_ms\": %.4f, \"p99_ms\": %.4f, "
+           "\"p50_ms_le64k\": %.4f, \"p99_ms_le64k\": %.4f, \"max_ms\": %.4f, "
+           "\"launches\": %lld, \"mean_requests_per_launch\": %.2f, \"books\": %d, "
+           "\"catalog_options\": %lld, \"greeks_frac\": %.2f, \"arena_peak_bytes\": %lld, "
+           "\"admission_waits\": %lld, \"use_graph\": %d}\n",
+           o.saturate ? "closed-loop saturation" : "open-loop poisson", o.saturate ? 0.0 : o.rate,
+           wall, (long long)requests_done, (long long)options_done, (double)options_done / wall,
+           pct(lat_all, 0.50), pct(lat_all, 0.99), pct(lat_small, 0.50), pct(lat_small, 0.99),
+           lat_all.empty() ? 0.0 : *std::max_element(lat_all.begin(), lat_all.end()),
+           (long long)st.launches,
+           st.launches ? (double)requests_done / (double)st.launches : 0.0, o.books,
+           (long long)catalog_options, o.greeks_frac, (long long)st.arena_peak,
+           (long long)st.admission_waits, o.use_graph);
+    return 0;
+}
