@@ -39,6 +39,22 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` on the n-option book, from the committed
+    rocprofv3 counter runs (profiles/hbm_traffic.json: FETCH_SIZE x 2 +
+    WRITE_SIZE x 1, factors calibrated on a plain 256 MiB copy).  None when
+    no profile of this size is present."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("options") != n:
+        return None
+    return rec["traffic_bytes"]
+
+
 def timed_passes(launch, nbooks: int, steps: int, warmup: int, stream) -> float:
     """Runs warmup + steps launches (rotating books); returns ms per step of the
     timed steps, from HIP events on the launch stream."""
@@ -140,8 +156,8 @@ def only_one(args, L, host, dbooks, outs, stream) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--books", type=int, default=4)
     ap.add_argument("--cpu-passes", type=int, default=20)
@@ -204,7 +220,8 @@ def main() -> int:
                    "options": n, "rotating_books": nb, "bytes_per_option": 25,
                    "max_abs_err_vs_oracle_sample": max_err},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("price", n)},
     }
 
     if not args.no_extra:

@@ -20,7 +20,10 @@ i64 = ctypes.c_int64
 @pytest.fixture(scope="module")
 def L():
     subprocess.run(["make", "-s", "-C", NATIVE], check=True)
-    lib = ctypes.CDLL(os.path.join(NATIVE, "libserve_core_shim.so"))
+    # BS_SERVE_CORE_SHIM=libserve_core_shim_asan.so (make -C tests/native asan, run
+    # with libasan preloaded) runs the same tests under AddressSanitizer + UBSan.
+    lib = ctypes.CDLL(os.path.join(NATIVE, os.environ.get("BS_SERVE_CORE_SHIM",
+                                                          "libserve_core_shim.so")))
     vp = ctypes.c_void_p
     P = ctypes.POINTER
     sig = {
