@@ -94,14 +94,17 @@ def cpu_baseline(book_host, passes: int) -> dict:
 def serve_bench(result: dict) -> dict:
     """Runs the serving load generator (p3arsec_amd/bin/bs_serve_bench, a C++
     client of the C ABI) as a child process: open-loop Poisson at 20K req/s
-    and a closed-loop saturation run; compares the saturated options/s with
-    the batch kernels' options/s for the same 70/30 price/Greeks mix."""
+    with direct launches (the default) and with graph replay, and a
+    closed-loop saturation run (4096 requests outstanding); compares the
+    saturated options/s with the batch kernels' options/s for the same 70/30
+    price/Greeks mix."""
     import subprocess
     exe = os.path.join(ROOT, "p3arsec_amd", "bin", "bs_serve_bench")
     secs = str(ARGS.serve_seconds)
     out = {}
     for tag, extra in (("poisson_20k", ["--rate", "20000"]),
-                       ("saturation", ["--saturate", "--inflight", "512"])):
+                       ("poisson_20k_graph", ["--rate", "20000", "--use-graph", "1"]),
+                       ("saturation", ["--saturate", "--inflight", "4096"])):
         p = subprocess.run([exe, "--duration-s", secs, "--budget-gb", "32"] + extra,
                            capture_output=True, text=True, timeout=300)
         if p.returncode != 0:

@@ -168,9 +168,9 @@ int bs_event_elapsed_ms(bs_event_t start, bs_event_t stop, float *ms);
  * (BS:664-960).  A context owns one HBM arena of `hbm_budget_bytes`; books
  * registered into it stay resident; requests against resident books are
  * queued, admitted when their output buffers fit in the arena, fused with the
- * other queued requests into one persistent-grid launch replayed from a
- * captured hipGraph, and completed in arrival order of their launch.
- * A context is thread-safe. */
+ * other queued requests into one batch (a fused kernel per request kind,
+ * launched directly or replayed from a captured hipGraph), and completed in
+ * the order their batches finish.  A context is thread-safe. */
 typedef struct bs_ctx_s *bs_ctx_t;
 typedef int64_t bs_book_t;
 typedef int64_t bs_ticket_t;
@@ -187,15 +187,29 @@ enum {
 typedef struct {
     int64_t max_batch_requests; /* requests fused per launch (<= 4096)       */
     int64_t max_batch_options;  /* options fused per launch                  */
-    int32_t use_graph;          /* 1: replay a captured hipGraph per launch  */
+    int32_t use_graph;          /* 1: replay a captured hipGraph per batch;
+                                   0: direct launches (default, faster)     */
     int32_t grid_blocks;        /* persistent grid size, 0 = auto            */
+    int32_t completion_queue;   /* 1: finished requests queue for bs_reap    */
+    int32_t reserved;           /* 0                                         */
 } bs_serve_opts;
+
+/* One finished request from the completion queue.  Times are steady-clock
+ * (CLOCK_MONOTONIC) nanoseconds. */
+typedef struct {
+    bs_ticket_t ticket;
+    int32_t state;  /* BS_TICKET_DONE, BS_TICKET_FAILED or BS_TICKET_CANCELLED */
+    int32_t status; /* BS_OK or the request's error code                     */
+    int64_t submit_ns, done_ns;
+} bs_completion;
 
 typedef struct {
     int64_t arena_bytes, arena_used, arena_peak;
     int64_t books_resident, requests_queued, requests_running;
     int64_t requests_done, requests_failed, requests_cancelled;
     int64_t launches, options_priced, admission_waits;
+    int64_t queue_ns_total; /* sum over admitted requests of submit -> launch  */
+    int64_t batch_ns_total; /* sum over batches of launch -> completion seen   */
 } bs_serve_stats;
 
 int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts *opts,
@@ -209,6 +223,16 @@ int bs_book_register(bs_ctx_t ctx, const float *s, const float *k, const float *
 /* Books still referenced by queued or running requests: BS_ERR_BUSY. */
 int bs_book_release(bs_ctx_t ctx, bs_book_t book);
 int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t *ticket);
+/* Submits count requests under one lock, all or none: an unknown book, a bad
+ * kind or outputs that can never fit fail the call before any is queued.
+ * Requests of one call join the queue in order. */
+int bs_submit_many(bs_ctx_t ctx, const bs_book_t *books, const int *kinds, int64_t count,
+                   bs_ticket_t *tickets);
+/* With opts.completion_queue set: pops up to max finished requests in the
+ * order they finished, waiting up to timeout_us for the first one (0 polls,
+ * < 0 waits forever).  *count = 0 after a timeout.  Outputs stay resident
+ * until released.  BS_ERR_INVALID when the context has no completion queue. */
+int bs_reap(bs_ctx_t ctx, bs_completion *out, int64_t max, int64_t timeout_us, int64_t *count);
 int bs_poll(bs_ctx_t ctx, bs_ticket_t ticket, int *state);
 /* timeout_us < 0 waits forever. */
 int bs_wait(bs_ctx_t ctx, bs_ticket_t ticket, int64_t timeout_us);
@@ -225,6 +249,9 @@ int bs_result_release(bs_ctx_t ctx, bs_ticket_t ticket);
 /* Cancels a queued request (running ones finish). */
 int bs_cancel(bs_ctx_t ctx, bs_ticket_t ticket);
 /* Nanosecond timestamps (CLOCK_MONOTONIC) of submit and completion. */
+/* Releases count tickets under one lock.  Every ticket is tried; the return
+ * value is the first failure (BS_ERR_NOT_FOUND, BS_ERR_BUSY) or BS_OK. */
+int bs_result_release_many(bs_ctx_t ctx, const bs_ticket_t *tickets, int64_t count);
 int bs_ticket_times(bs_ctx_t ctx, bs_ticket_t ticket, int64_t *submit_ns,
                     int64_t *done_ns);
 int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats *stats);

@@ -48,14 +48,21 @@ _f32 = ctypes.c_float
 
 class ServeOpts(ctypes.Structure):
     _fields_ = [("max_batch_requests", _i64), ("max_batch_options", _i64),
-                ("use_graph", ctypes.c_int32), ("grid_blocks", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("grid_blocks", ctypes.c_int32),
+                ("completion_queue", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("ticket", _i64), ("state", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("submit_ns", _i64), ("done_ns", _i64)]
 
 
 class ServeStats(ctypes.Structure):
     _fields_ = [(n, _i64) for n in (
         "arena_bytes", "arena_used", "arena_peak", "books_resident", "requests_queued",
         "requests_running", "requests_done", "requests_failed", "requests_cancelled",
-        "launches", "options_priced", "admission_waits")]
+        "launches", "options_priced", "admission_waits", "queue_ns_total",
+        "batch_ns_total")]
 
 
 # name -> (argtypes, restype); restype int unless given
@@ -100,6 +107,10 @@ _SIGS = {
     "bs_book_register": ([_vp] + [_vp] * 6 + [_i64, ctypes.POINTER(_i64)], _i32),
     "bs_book_release": ([_vp, _i64], _i32),
     "bs_submit": ([_vp, _i64, _i32, ctypes.POINTER(_i64)], _i32),
+    "bs_submit_many": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int), _i64,
+                        ctypes.POINTER(_i64)], _i32),
+    "bs_reap": ([_vp, ctypes.POINTER(Completion), _i64, _i64, ctypes.POINTER(_i64)], _i32),
+    "bs_result_release_many": ([_vp, ctypes.POINTER(_i64), _i64], _i32),
     "bs_poll": ([_vp, _i64, ctypes.POINTER(_i32)], _i32),
     "bs_wait": ([_vp, _i64, _i64], _i32),
     "bs_result_copy": ([_vp, _i64] + [_vp] * 5, _i32),

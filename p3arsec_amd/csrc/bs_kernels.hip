@@ -16,266 +16,10 @@
 
 #include "bs_internal.hpp"
 #include "bs_math.hpp"
+#include "bs_ops.hpp"
 
 namespace bs {
 namespace {
-
-template <class T, int N>
-using vec = T __attribute__((ext_vector_type(N)));
-
-// VW type bytes as one integer: uint16_t for 2 options, uint32_t for 4.
-template <int VW>
-using tword = typename std::conditional<VW == 4, uint32_t, uint16_t>::type;
-
-template <bool NT, class T>
-__device__ __forceinline__ T ld(const T* p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <bool NT, class T>
-__device__ __forceinline__ void st(T* p, T v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
-__device__ __forceinline__ bool put_of(uint32_t word, int j) {
-    return ((word >> (8 * j)) & 0xffu) != 0u;
-}
-
-// ---- per-group operations --------------------------------------------------
-// Each op: Group (registers of one VW-option group), load<NT>(g),
-// finish<NT>(g, x) (compute + store), scalar(i) for options outside whole
-// groups, kWaves (the occupancy the register allocator is held to: 8 waves
-// per SIMD = 64 VGPRs), kBlocks (default grid cap, 0 = one group per
-// work-item) and kPipe (default loop form); the defaults and each launcher's
-// preferred width are the fastest forms measured on MI355X for 10M options.
-
-template <class Math, int VW>
-struct PriceF32 {
-    static constexpr int kVW = VW;
-    static constexpr int kWaves = 8;
-    static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = true;
-    using F = vec<float, VW>;
-    SoaIn<float> in;
-    const uint8_t* type;
-    float* out;
-    struct Group {
-        F s, k, r, v, t;
-        tword<VW> ty;
-    };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        Group x;
-        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const tword<VW>*>(type) + g);
-        return x;
-    }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        F p;
-#pragma unroll
-        for (int j = 0; j < VW; ++j)
-            p[j] = bs_price<Math>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
-        st<NT>(reinterpret_cast<F*>(out) + g, p);
-    }
-    __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = bs_price<Math>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
-    }
-};
-
-template <int VW>
-struct PriceI32F32 {
-    static constexpr int kVW = VW;
-    static constexpr int kWaves = 7;
-    static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = true;
-    using F = vec<float, VW>;
-    using I = vec<int, VW>;
-    SoaIn<float> in;
-    const int32_t* type;
-    float* out;
-    struct Group {
-        F s, k, r, v, t;
-        I ty;
-    };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        Group x;
-        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const I*>(type) + g);
-        return x;
-    }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        F p;
-#pragma unroll
-        for (int j = 0; j < VW; ++j)
-            p[j] = bs_price<Ref32>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], x.ty[j] != 0);
-        st<NT>(reinterpret_cast<F*>(out) + g, p);
-    }
-    __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = bs_price<Ref32>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
-    }
-};
-
-// fptype = double: a group is VW = 2 options (one 128-bit load per array).
-template <int VW>
-struct PriceF64 {
-    static constexpr int kVW = 2;
-    static constexpr int kWaves = 5;
-    static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = true;
-    using D = vec<double, 2>;
-    SoaIn<double> in;
-    const uint8_t* type;
-    double* out;
-    struct Group {
-        D s, k, r, v, t;
-        uint16_t ty;
-    };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        Group x;
-        x.s = ld<NT>(reinterpret_cast<const D*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const D*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const D*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const D*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const D*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const uint16_t*>(type) + g);
-        return x;
-    }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        D p;
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            p[j] = bs_price<Exact64>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
-        st<NT>(reinterpret_cast<D*>(out) + g, p);
-    }
-    __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = bs_price<Exact64>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
-    }
-};
-
-template <int VW>
-struct GreeksF32 {
-    static constexpr int kVW = VW;
-    static constexpr int kWaves = VW == 4 ? 5 : 8;
-    static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = false;
-    using F = vec<float, VW>;
-    SoaIn<float> in;
-    const uint8_t* type;
-    GreeksOut out;
-    struct Group {
-        F s, k, r, v, t;
-        tword<VW> ty;
-    };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        Group x;
-        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const tword<VW>*>(type) + g);
-        return x;
-    }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
-        F p, d, ve, rh, th;
-#pragma unroll
-        for (int j = 0; j < VW; ++j) {
-            const Greeks32 o = bs_greeks<Ref32>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j],
-                                                put_of(x.ty, j));
-            p[j] = o.price;
-            d[j] = o.delta;
-            ve[j] = o.vega;
-            rh[j] = o.rho;
-            th[j] = o.theta;
-        }
-        st<NT>(reinterpret_cast<F*>(out.price) + g, p);
-        st<NT>(reinterpret_cast<F*>(out.delta) + g, d);
-        st<NT>(reinterpret_cast<F*>(out.vega) + g, ve);
-        st<NT>(reinterpret_cast<F*>(out.rho) + g, rh);
-        st<NT>(reinterpret_cast<F*>(out.theta) + g, th);
-    }
-    __device__ __forceinline__ void scalar(int64_t i) const {
-        const Greeks32 o = bs_greeks<Ref32>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i],
-                                            type[i] != 0);
-        out.price[i] = o.price;
-        out.delta[i] = o.delta;
-        out.vega[i] = o.vega;
-        out.rho[i] = o.rho;
-        out.theta[i] = o.theta;
-    }
-};
-
-template <int VW>
-struct CndfF32 {
-    static constexpr int kVW = VW;
-    static constexpr int kWaves = 8;
-    static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = false;
-    using F = vec<float, VW>;
-    const float* x;
-    float* out;
-    struct Group {
-        F x;
-    };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        return Group{ld<NT>(reinterpret_cast<const F*>(x) + g)};
-    }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
-        F y;
-#pragma unroll
-        for (int j = 0; j < VW; ++j) y[j] = Ref32::cndf<false>(a.x[j], nullptr);
-        st<NT>(reinterpret_cast<F*>(out) + g, y);
-    }
-    __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = Ref32::cndf<false>(x[i], nullptr);
-    }
-};
-
-template <int VW>
-struct CndfF64 {
-    static constexpr int kVW = 2;
-    static constexpr int kWaves = 8;
-    static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = false;
-    using D = vec<double, 2>;
-    const double* x;
-    double* out;
-    struct Group {
-        D x;
-    };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        return Group{ld<NT>(reinterpret_cast<const D*>(x) + g)};
-    }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
-        D y;
-        y[0] = Exact64::cndf<false>(a.x[0], nullptr);
-        y[1] = Exact64::cndf<false>(a.x[1], nullptr);
-        st<NT>(reinterpret_cast<D*>(out) + g, y);
-    }
-    __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = Exact64::cndf<false>(x[i], nullptr);
-    }
-};
 
 // ---- argument packs and their op constructors ----------------------------------
 struct PriceArgs {
@@ -325,11 +69,6 @@ Op make_op(const CndfArgs<double>& a) { return Op{a.x, a.out}; }
 // (register double buffering), so a wave overlaps its own HBM latency with its
 // arithmetic; it holds one more group of registers, so its occupancy target
 // is two waves lower.
-template <class Op, bool PIPE>
-constexpr int waves_for() {
-    return PIPE ? (Op::kWaves > 5 ? Op::kWaves - 2 : 3) : Op::kWaves;
-}
-
 template <class Op, bool NT, bool PIPE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, PIPE>())))
 void k_stream(Op op, int64_t ngroups, int64_t tail_begin, int64_t n) {

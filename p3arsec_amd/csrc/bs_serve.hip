@@ -1,22 +1,24 @@
 // bs_serve.hip -- the serving loop: resident books, continuous batching,
-// persistent fused launches replayed from captured hipGraphs.
+// fused launches of every queued request.
 //
 // A context owns
 //   * one device arena (hipMalloc of the HBM budget) holding resident books
 //     and request outputs, carved by serve::Core's first-fit allocator;
-//   * two descriptor slots, each a pinned host table plus its device copy;
-//   * per slot, one hipGraph: a stage kernel that copies the slot's table from
-//     pinned host memory to the device, then the fused persistent kernel that
-//     prices every queued request of the batch (price or price + Greeks) over
-//     a fixed grid.  The graphs never change shape; a batch's size lives in the
-//     table header, so each launch is one hipGraphLaunch;
+//   * four descriptor slots, each a pinned host table plus its device copy;
+//   * per batch, a stage kernel that copies the slot's table from pinned host
+//     memory to the device, then one fused kernel per section (the batch's
+//     price requests, its Greeks requests): a block prices one tile of 512
+//     options of one request, found by a 64-ary wave search of the table.
+//     Launched directly (the default) each section's grid is one block per
+//     tile; with use_graph the three kernels are captured once per slot over
+//     a fixed persistent grid and each batch is one hipGraphLaunch (on ROCm
+//     7.2 the direct form measured 15 us launch-to-completion against 31 us);
 //   * a dispatcher thread: it admits queued requests as arena memory allows,
-//     writes a table into a free slot and launches that slot's graph behind the
-//     one still running (two batches in flight), and retires a batch when its
-//     event completes.  Requests that arrive while a batch runs go into the
-//     next one.
+//     writes a table into a free slot and launches it behind the batch still
+//     running (up to four batches in flight), and retires a batch when its event
+//     completes.  Requests that arrive while a batch runs go into the next one.
 // Results are bit-identical to bs_price_f32 / bs_price_greeks_f32: the fused
-// kernel calls the same bs_price<Ref32> and bs_greeks<Ref32>.
+// kernels run the batch map's own per-group code (bs_ops.hpp).
 #include <string.h>
 #include <time.h>
 
@@ -31,16 +33,26 @@
 
 #include "bs_internal.hpp"
 #include "bs_math.hpp"
+#include "bs_ops.hpp"
 #include "serve_core.hpp"
 
 namespace bs {
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-constexpr int kTile = kThreads * 4;   // options per block step: 256 threads x 4
-constexpr int kMaxDesc = 4096;
+constexpr int kVW = 2;                 // options per work-item (the batch map's width)
+constexpr int kTile = kThreads * kVW;  // options per tile: one block step
+constexpr int kMaxDesc = 4096;         // requests per batch: 64 x 64 for find_desc
+constexpr int64_t kMaxGrid = int64_t(1) << 20;  // direct-launch grid cap (blocks loop)
 constexpr int kStageBlocks = 32;
+// Batches in flight at once.  While one runs, the next ones are queued behind
+// it on the stream, so the GPU stays busy while clients turn finished
+// requests around into new ones.
+constexpr int kSlots = 4;
+constexpr int kReapSpinUs = 200;
+// With fewer than two batches on the GPU and more than this many options
+// queued, the dispatcher admits half of the queue so that the other half's
+// launch (and the clients' turnaround of the first half) overlaps the first.
+constexpr int64_t kSplitOptions = int64_t(4) << 20;
 
 // One request of a batch: 112 bytes, a multiple of the stage kernel's 16-byte words.
 struct Desc {
@@ -48,113 +60,89 @@ struct Desc {
     const uint8_t* type;
     float* out[5];        // price, delta, vega, rho, theta (Greeks only use 1..4)
     int64_t n;
-    int64_t tile_begin;   // first global tile of this request
+    int64_t tile_begin;   // first tile of this request within its section
     int32_t kind;
     int32_t pad;
 };
 static_assert(sizeof(Desc) == 112, "Desc layout");
 
+// A batch's table: the header, then the price requests' descriptors, then the
+// Greeks requests' descriptors.  Section 0 is price, section 1 Greeks.
 struct SlotHeader {
-    int32_t ndesc;
-    int32_t pad;
-    int64_t total_tiles;
+    int32_t ndesc[2];
+    int64_t tiles[2];
     int64_t seq;
-    int64_t pad2[5];
+    int64_t pad[4];
 };
 static_assert(sizeof(SlotHeader) == 64, "SlotHeader layout");
 
 constexpr size_t kSlotBytes = sizeof(SlotHeader) + sizeof(Desc) * kMaxDesc;
 
-// Copies header + ndesc descriptors from the pinned table to device memory.
+// Copies header + descriptors from the pinned table to device memory.
 __global__ __launch_bounds__(kThreads) void k_stage(const uint4* __restrict__ host,
                                                     uint4* __restrict__ dev) {
     const SlotHeader* h = reinterpret_cast<const SlotHeader*>(host);
-    const int ndesc = h->ndesc < 0 ? 0 : (h->ndesc > kMaxDesc ? kMaxDesc : h->ndesc);
+    const int np = h->ndesc[0], ng = h->ndesc[1];
+    const int ndesc = (np < 0 || ng < 0 || np + ng > kMaxDesc) ? 0 : np + ng;
     const int64_t words = (int64_t)(sizeof(SlotHeader) + sizeof(Desc) * ndesc) / 16;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < words;
          i += (int64_t)gridDim.x * kThreads)
         dev[i] = host[i];
 }
 
-__device__ __forceinline__ bool put_of(uint32_t w, int j) { return ((w >> (8 * j)) & 0xffu) != 0u; }
-
-// Prices options [o0, o0 + kTile) of one request (clipped to n).
-__device__ __forceinline__ void run_tile(const Desc& d, int64_t o0) {
-    const int64_t g = o0 / 4 + threadIdx.x;   // 4-option group of this thread
-    const int64_t i0 = g * 4;
-    if (i0 >= d.n) return;
-    const bool greeks = d.kind == serve::kGreeks;
-    if (i0 + 4 <= d.n) {
-        // arena blocks are 256-byte aligned: the group loads are 16-byte aligned
-        const f4 s = __builtin_nontemporal_load(reinterpret_cast<const f4*>(d.in[0]) + g);
-        const f4 k = __builtin_nontemporal_load(reinterpret_cast<const f4*>(d.in[1]) + g);
-        const f4 r = __builtin_nontemporal_load(reinterpret_cast<const f4*>(d.in[2]) + g);
-        const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(d.in[3]) + g);
-        const f4 t = __builtin_nontemporal_load(reinterpret_cast<const f4*>(d.in[4]) + g);
-        const uint32_t ty = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(d.type) + g);
-        if (!greeks) {
-            f4 p;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) p[j] = bs_price<Ref32>(s[j], k[j], r[j], v[j], t[j], put_of(ty, j));
-            __builtin_nontemporal_store(p, reinterpret_cast<f4*>(d.out[0]) + g);
-        } else {
-            f4 o[5];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const Greeks32 q = bs_greeks<Ref32>(s[j], k[j], r[j], v[j], t[j], put_of(ty, j));
-                o[0][j] = q.price;
-                o[1][j] = q.delta;
-                o[2][j] = q.vega;
-                o[3][j] = q.rho;
-                o[4][j] = q.theta;
-            }
-#pragma unroll
-            for (int q = 0; q < 5; ++q)
-                __builtin_nontemporal_store(o[q], reinterpret_cast<f4*>(d.out[q]) + g);
-        }
-        return;
-    }
-    for (int64_t i = i0; i < d.n; ++i) {
-        const float s = d.in[0][i], k = d.in[1][i], r = d.in[2][i], v = d.in[3][i], t = d.in[4][i];
-        const bool put = d.type[i] != 0;
-        if (!greeks) {
-            d.out[0][i] = bs_price<Ref32>(s, k, r, v, t, put);
-        } else {
-            const Greeks32 q = bs_greeks<Ref32>(s, k, r, v, t, put);
-            d.out[0][i] = q.price;
-            d.out[1][i] = q.delta;
-            d.out[2][i] = q.vega;
-            d.out[3][i] = q.rho;
-            d.out[4][i] = q.theta;
-        }
-    }
+// The request of tile t among d[0, nd): the last descriptor whose tile_begin
+// <= t.  A 64-ary search over tile_begin (increasing, d[0].tile_begin == 0):
+// each wave resolves 64 x 64 = 4096 descriptors with two loads per lane.
+__device__ __forceinline__ int find_desc(const Desc* d, int nd, int64_t t) {
+    const int lane = threadIdx.x & 63;
+    int j = lane * 64;
+    const int base = (__popcll(__ballot(j < nd && d[j].tile_begin <= t)) - 1) * 64;
+    j = base + lane;
+    return base + __popcll(__ballot(j < nd && d[j].tile_begin <= t)) - 1;
 }
 
-// Fused persistent kernel: block b takes the contiguous tile range
-// [b*T/G, (b+1)*T/G) of the batch's T tiles, finds the first tile's request
-// by binary search over tile_begin and walks forward from there.
-__global__ __launch_bounds__(kThreads) void k_fused(const uint8_t* __restrict__ slot) {
+template <class Op>
+__device__ __forceinline__ Op op_of(const Desc& x);
+template <>
+__device__ __forceinline__ PriceF32<Ref32, kVW> op_of(const Desc& x) {
+    return {SoaIn<float>{x.in[0], x.in[1], x.in[2], x.in[3], x.in[4]}, x.type, x.out[0]};
+}
+template <>
+__device__ __forceinline__ GreeksF32<kVW> op_of(const Desc& x) {
+    return {SoaIn<float>{x.in[0], x.in[1], x.in[2], x.in[3], x.in[4]}, x.type,
+            GreeksOut{x.out[0], x.out[1], x.out[2], x.out[3], x.out[4]}};
+}
+
+// Fused kernel of one section: a block prices tile t (kTile options of one
+// request, one kVW-group per work-item) for t = blockIdx.x, + gridDim.x, ...
+// Launched directly, the grid has one block per tile; replayed from a graph it
+// is a fixed persistent grid.  Every access is bounded by the request's n.
+// The per-group code is the batch map's own (bs_ops.hpp).
+template <class Op, int kSection>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, false>())))
+void k_serve(const uint8_t* __restrict__ slot) {
     const SlotHeader* h = reinterpret_cast<const SlotHeader*>(slot);
-    const Desc* desc = reinterpret_cast<const Desc*>(slot + sizeof(SlotHeader));
-    const int ndesc = h->ndesc;
-    const int64_t total = h->total_tiles;
-    if (ndesc <= 0 || ndesc > kMaxDesc || total <= 0) return;
-    const int64_t G = gridDim.x;
-    const int64_t t0 = total * blockIdx.x / G, t1 = total * (blockIdx.x + 1) / G;
-    if (t0 >= t1) return;
-    int lo = 0, hi = ndesc - 1;   // last desc with tile_begin <= t0
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (desc[mid].tile_begin <= t0) lo = mid;
-        else hi = mid - 1;
-    }
-    int di = lo;
-    for (int64_t t = t0; t < t1; ++t) {
-        while (di + 1 < ndesc && desc[di + 1].tile_begin <= t) ++di;
-        const Desc& d = desc[di];
-        run_tile(d, (t - d.tile_begin) * kTile);
+    const int np = h->ndesc[0], ng = h->ndesc[1];
+    if (np < 0 || ng < 0 || np + ng > kMaxDesc) return;
+    const int nd = kSection == 0 ? np : ng;
+    if (nd <= 0) return;
+    const Desc* d = reinterpret_cast<const Desc*>(slot + sizeof(SlotHeader)) + (kSection ? np : 0);
+    const int64_t tiles = h->tiles[kSection];
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const Desc& x = d[find_desc(d, nd, t)];
+        const Op op = op_of<Op>(x);
+        const int64_t n = x.n;
+        const int64_t g = (t - x.tile_begin) * kThreads + threadIdx.x;
+        if ((g + 1) * kVW <= n) {
+            op.template finish<true>(g, op.template load<true>(g));
+        } else {
+            for (int64_t i = g * kVW; i < n; ++i) op.scalar(i);
+        }
     }
 }
+
+using ServePrice = PriceF32<Ref32, kVW>;
+using ServeGreeks = GreeksF32<kVW>;
 
 int64_t now_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -165,12 +153,19 @@ int64_t now_ns() {
 struct Batch {
     int slot;
     hipEvent_t done;
+    int64_t t_launch;
     std::vector<int64_t> tickets;
 };
 
 }  // namespace
 }  // namespace bs
 
+// Locking: `mu` guards `core`, `fatal` and `batch_ns`.  The dispatcher takes it
+// only to admit a batch or retire one; while batches run it polls their
+// events with the lock released, so submitting and reaping threads never wait
+// behind the poll loop.  `gen` counts the changes that can unblock admission
+// (a submit, a freed result or book, a cancel, shutdown); `queued` mirrors
+// the queue length.  Both are written under `mu` and read without it.
 struct bs_ctx_s {
     int device = 0;
     bs_serve_opts opts{};
@@ -180,18 +175,26 @@ struct bs_ctx_s {
     bs::serve::Core* core = nullptr;
     char* arena = nullptr;
     hipStream_t stream = nullptr;
-    uint8_t* host_slot[2] = {nullptr, nullptr};    // pinned, written by the dispatcher
-    uint8_t* host_slot_dev[2] = {nullptr, nullptr};  // the same memory, device address
-    uint8_t* dev_slot[2] = {nullptr, nullptr};
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    bool slot_busy[2] = {false, false};
+    uint8_t* host_slot[bs::kSlots] = {};      // pinned, written by the dispatcher
+    uint8_t* host_slot_dev[bs::kSlots] = {};  // the same memory, device address
+    uint8_t* dev_slot[bs::kSlots] = {};
+    hipGraphExec_t graph[bs::kSlots] = {};
+    hipEvent_t ev[bs::kSlots] = {};
     int grid = 0;
     int64_t seq = 0;
-    std::vector<bs::Batch> inflight;
+    int64_t batch_ns = 0;              // sum of launch -> completion seen, over batches
+    std::atomic<int64_t> gen{0};
+    std::atomic<int64_t> queued{0};
+    std::atomic<int64_t> finished{0};  // requests finished so far (done, failed, cancelled)
+    std::atomic<bool> shutdown{false};
     std::thread worker;
-    bool shutdown = false;
     std::string fatal;
+
+    // Called under `mu` after any change to the queue or the arena.
+    void changed() {
+        queued.store((int64_t)core->queued(), std::memory_order_release);
+        gen.fetch_add(1, std::memory_order_acq_rel);
+    }
 };
 
 using namespace bs;
@@ -204,16 +207,25 @@ static_assert(serve::kErrBudget == BS_ERR_BUDGET && serve::kErrBusy == BS_ERR_BU
                   serve::kCancelled == BS_TICKET_CANCELLED && serve::kGreeks == BS_REQ_GREEKS,
               "serve_core codes mirror bs_mi355.h");
 
-// Launches one slot: graph replay or the two kernels directly.
+// Launches one slot: graph replay, or the stage kernel and the fused kernel of
+// each non-empty section directly, each section's grid one block per tile.
 hipError_t launch_slot(bs_ctx_s* c, int s) {
     if (c->graph[s]) return hipGraphLaunch(c->graph[s], c->stream);
+    const SlotHeader* h = reinterpret_cast<const SlotHeader*>(c->host_slot[s]);
     hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, c->stream,
                        reinterpret_cast<const uint4*>(c->host_slot_dev[s]),
                        reinterpret_cast<uint4*>(c->dev_slot[s]));
-    hipLaunchKernelGGL(k_fused, dim3(c->grid), dim3(kThreads), 0, c->stream, c->dev_slot[s]);
+    if (h->tiles[0] > 0)
+        hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3((unsigned)std::min(h->tiles[0], kMaxGrid)),
+                           dim3(kThreads), 0, c->stream, c->dev_slot[s]);
+    if (h->tiles[1] > 0)
+        hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3((unsigned)std::min(h->tiles[1], kMaxGrid)),
+                           dim3(kThreads), 0, c->stream, c->dev_slot[s]);
     return hipGetLastError();
 }
 
+// Captures slot s as one graph: stage, price section, Greeks section, each
+// fused kernel over the fixed persistent grid c->grid.
 hipError_t capture_slot(bs_ctx_s* c, int s) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
@@ -221,7 +233,10 @@ hipError_t capture_slot(bs_ctx_s* c, int s) {
     hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, c->stream,
                        reinterpret_cast<const uint4*>(c->host_slot_dev[s]),
                        reinterpret_cast<uint4*>(c->dev_slot[s]));
-    hipLaunchKernelGGL(k_fused, dim3(c->grid), dim3(kThreads), 0, c->stream, c->dev_slot[s]);
+    hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3(c->grid), dim3(kThreads), 0, c->stream,
+                       c->dev_slot[s]);
+    hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3(c->grid), dim3(kThreads), 0, c->stream,
+                       c->dev_slot[s]);
     e = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&c->graph[s], g, nullptr, nullptr, 0);
@@ -229,85 +244,125 @@ hipError_t capture_slot(bs_ctx_s* c, int s) {
     return e;
 }
 
-// Fills slot s with the batch's descriptors (caller holds the lock).
+// Fills slot s with the batch's descriptors, price requests first, each
+// section in admission order (caller holds the lock).
 void write_slot(bs_ctx_s* c, int s, const std::vector<serve::Request*>& reqs) {
     SlotHeader* h = reinterpret_cast<SlotHeader*>(c->host_slot[s]);
     Desc* d = reinterpret_cast<Desc*>(c->host_slot[s] + sizeof(SlotHeader));
-    int64_t tiles = 0;
-    for (size_t i = 0; i < reqs.size(); ++i) {
-        const serve::Request* r = reqs[i];
+    int nd[2] = {0, 0};
+    for (const serve::Request* r : reqs) ++nd[r->kind == serve::kGreeks ? 1 : 0];
+    int next[2] = {0, nd[0]};
+    int64_t tiles[2] = {0, 0};
+    for (const serve::Request* r : reqs) {
+        const int sec = r->kind == serve::kGreeks ? 1 : 0;
         const serve::BookRec* b = c->core->book(r->book);
         size_t bo[6], oo[5];
         serve::book_layout(b->n, bo);
         serve::output_layout(r->n, r->kind, oo);
-        Desc& x = d[i];
+        Desc& x = d[next[sec]++];
         for (int j = 0; j < 5; ++j) x.in[j] = reinterpret_cast<const float*>(c->arena + b->off + bo[j]);
         x.type = reinterpret_cast<const uint8_t*>(c->arena + b->off + bo[5]);
         for (int j = 0; j < 5; ++j) x.out[j] = reinterpret_cast<float*>(c->arena + r->out_off + oo[j]);
         x.n = r->n;
-        x.tile_begin = tiles;
+        x.tile_begin = tiles[sec];
         x.kind = r->kind;
         x.pad = 0;
-        tiles += (r->n + kTile - 1) / kTile;
+        tiles[sec] += (r->n + kTile - 1) / kTile;
     }
-    h->ndesc = (int32_t)reqs.size();
-    h->total_tiles = tiles;
+    h->ndesc[0] = nd[0];
+    h->ndesc[1] = nd[1];
+    h->tiles[0] = tiles[0];
+    h->tiles[1] = tiles[1];
     h->seq = ++c->seq;
 }
 
 void dispatcher(bs_ctx_s* c) {
     (void)hipSetDevice(c->device);
-    std::unique_lock<std::mutex> lk(c->mu);
+    std::vector<Batch> inflight;        // launch order; owned by this thread
+    bool busy[kSlots] = {};
+    int64_t blocked_gen = -1;           // gen at the last admission that found no room
     for (;;) {
         bool progressed = false;
         // retire finished batches in launch order
-        while (!c->inflight.empty()) {
-            Batch& b = c->inflight.front();
-            lk.unlock();
+        while (!inflight.empty()) {
+            Batch& b = inflight.front();
             const hipError_t q = hipEventQuery(b.done);
-            lk.lock();
             if (q == hipErrorNotReady) break;
-            const bool ok = q == hipSuccess;
-            if (!ok) c->fatal = std::string("batch failed: ") + hipGetErrorString(q);
-            c->core->complete(b.tickets, ok, now_ns());
-            c->slot_busy[b.slot] = false;
-            c->inflight.erase(c->inflight.begin());
-            progressed = true;
+            const int64_t t = now_ns();
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                if (q != hipSuccess) c->fatal = std::string("batch failed: ") + hipGetErrorString(q);
+                c->core->complete(b.tickets, q == hipSuccess, t);
+                c->batch_ns += t - b.t_launch;
+                c->changed();
+                c->finished.fetch_add((int64_t)b.tickets.size(), std::memory_order_release);
+            }
             c->cv_done.notify_all();
+            busy[b.slot] = false;
+            inflight.erase(inflight.begin());
+            blocked_gen = -1;   // a failed batch frees its outputs
+            progressed = true;
         }
         // launch the next batch into a free slot
-        int s = !c->slot_busy[0] ? 0 : (!c->slot_busy[1] ? 1 : -1);
-        if (s >= 0 && !c->shutdown && c->core->queued() > 0) {
-            std::vector<serve::Request*> reqs = c->core->admit(now_ns());
+        int s = -1;
+        for (int j = 0; j < kSlots && s < 0; ++j)
+            if (!busy[j]) s = j;
+        if (s >= 0 && !c->shutdown.load(std::memory_order_acquire) &&
+            c->queued.load(std::memory_order_acquire) > 0 &&
+            c->gen.load(std::memory_order_acquire) != blocked_gen) {
+            std::vector<serve::Request*> reqs;
+            Batch b{s, c->ev[s], 0, {}};
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                const int64_t g = c->gen.load(std::memory_order_relaxed);
+                const int64_t qo = c->core->queued_options();
+                reqs = c->core->admit(now_ns(), inflight.size() < 2 && qo > kSplitOptions ? qo / 2 : 0);
+                if (reqs.empty()) {
+                    blocked_gen = g;   // the queue head does not fit: wait for a change
+                } else {
+                    write_slot(c, s, reqs);
+                    for (auto* r : reqs) b.tickets.push_back(r->ticket);
+                    c->changed();
+                }
+            }
             if (!reqs.empty()) {
-                write_slot(c, s, reqs);
-                Batch b{s, c->ev[s], {}};
-                for (auto* r : reqs) b.tickets.push_back(r->ticket);
+                b.t_launch = now_ns();
                 hipError_t e = launch_slot(c, s);
                 if (e == hipSuccess) e = hipEventRecord(c->ev[s], c->stream);
                 if (e != hipSuccess) {
-                    c->fatal = std::string("launch failed: ") + hipGetErrorString(e);
-                    c->core->complete(b.tickets, false, now_ns());
+                    {
+                        std::lock_guard<std::mutex> lk(c->mu);
+                        c->fatal = std::string("launch failed: ") + hipGetErrorString(e);
+                        c->core->complete(b.tickets, false, now_ns());
+                        c->changed();
+                        c->finished.fetch_add((int64_t)b.tickets.size(), std::memory_order_release);
+                    }
                     c->cv_done.notify_all();
                 } else {
-                    c->slot_busy[s] = true;
-                    c->inflight.push_back(std::move(b));
+                    busy[s] = true;
+                    inflight.push_back(std::move(b));
                 }
                 progressed = true;
             }
         }
-        if (c->shutdown && c->inflight.empty()) break;
         if (progressed) continue;
-        if (!c->inflight.empty()) {
-            // a batch is on the GPU: poll its event without holding the lock
-            lk.unlock();
-            std::this_thread::yield();
-            lk.lock();
-        } else {
-            c->cv_work.wait(lk);
+        if (!inflight.empty()) {
+            std::this_thread::yield();   // a batch is on the GPU: keep polling its event
+            continue;
         }
+        if (c->shutdown.load(std::memory_order_acquire)) break;
+        std::unique_lock<std::mutex> lk(c->mu);
+        c->cv_work.wait(lk, [&] {
+            return c->shutdown.load(std::memory_order_relaxed) ||
+                   (c->core->queued() > 0 && c->gen.load(std::memory_order_relaxed) != blocked_gen);
+        });
     }
-    c->core->fail_queued(serve::kErrShutdown, now_ns());
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->core->fail_queued(serve::kErrShutdown, now_ns());
+        c->changed();
+        c->finished.fetch_add(1, std::memory_order_release);
+    }
     c->cv_done.notify_all();
 }
 
@@ -328,7 +383,7 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
     c->device = device;
     c->opts.max_batch_requests = kMaxDesc;
     c->opts.max_batch_options = int64_t(1) << 31;
-    c->opts.use_graph = 1;
+    c->opts.use_graph = 0;
     c->opts.grid_blocks = 0;
     if (opts) {
         if (opts->max_batch_requests > 0)
@@ -341,7 +396,7 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipMalloc((void**)&c->arena, hbm_budget_bytes);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int s = 0; s < 2 && e == hipSuccess; ++s) {
+    for (int s = 0; s < kSlots && e == hipSuccess; ++s) {
         e = hipHostMalloc((void**)&c->host_slot[s], kSlotBytes, hipHostMallocCoherent);
         if (e == hipSuccess)
             e = hipHostGetDevicePointer((void**)&c->host_slot_dev[s], c->host_slot[s], 0);
@@ -354,7 +409,7 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
         c->grid = c->opts.grid_blocks > 0 ? c->opts.grid_blocks : std::max(1, cus) * 8;
     }
-    for (int s = 0; s < 2 && e == hipSuccess && c->opts.use_graph; ++s) e = capture_slot(c, s);
+    for (int s = 0; s < kSlots && e == hipSuccess && c->opts.use_graph; ++s) e = capture_slot(c, s);
     if (e != hipSuccess) {
         rc = hip_error(e, "bs_ctx_create");
         bs_ctx_destroy(c);
@@ -364,6 +419,7 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
     lim.max_batch_requests = c->opts.max_batch_requests;
     lim.max_batch_options = c->opts.max_batch_options;
     c->core = new serve::Core(hbm_budget_bytes, lim);
+    c->core->enable_completions(opts && opts->completion_queue != 0);
     c->worker = std::thread(dispatcher, c);
     *out = c;
     return BS_OK;
@@ -375,14 +431,15 @@ int bs_ctx_destroy(bs_ctx_t ctx) {
     if (c->worker.joinable()) {
         {
             std::lock_guard<std::mutex> lk(c->mu);
-            c->shutdown = true;
+            c->shutdown.store(true);
+            c->gen.fetch_add(1);
         }
         c->cv_work.notify_all();
         c->worker.join();
     }
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kSlots; ++s) {
         if (c->graph[s]) (void)hipGraphExecDestroy(c->graph[s]);
         if (c->ev[s]) (void)hipEventDestroy(c->ev[s]);
         if (c->host_slot[s]) (void)hipHostFree(c->host_slot[s]);
@@ -405,7 +462,7 @@ int bs_book_register(bs_ctx_t ctx, const float* s, const float* k, const float* 
     size_t off;
     {
         std::lock_guard<std::mutex> lk(c->mu);
-        if (c->shutdown) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
+        if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
         const int rc = c->core->register_book(n, &id, &off);
         if (rc != BS_OK) return set_error(rc, "book does not fit in the arena");
     }
@@ -419,6 +476,7 @@ int bs_book_register(bs_ctx_t ctx, const float* s, const float* k, const float* 
     if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk(c->mu);
         c->core->release_book(id);
+        c->changed();
         return hip_error(e, "bs_book_register");
     }
     *book = id;
@@ -432,6 +490,7 @@ int bs_book_release(bs_ctx_t ctx, bs_book_t book) {
     {
         std::lock_guard<std::mutex> lk(c->mu);
         rc = c->core->release_book(book);
+        if (rc == BS_OK) c->changed();
     }
     if (rc == BS_OK) c->cv_work.notify_all();
     return rc == BS_OK ? BS_OK : set_error(rc, "bs_book_release");
@@ -443,12 +502,92 @@ int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t* ticket) {
     int rc;
     {
         std::lock_guard<std::mutex> lk(c->mu);
-        if (c->shutdown) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
+        if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
         rc = c->core->submit(book, kind, now_ns(), ticket);
+        if (rc == BS_OK) c->changed();
     }
     if (rc != BS_OK) return set_error(rc, "bs_submit");
     c->cv_work.notify_one();
     return BS_OK;
+}
+
+int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, int64_t count,
+                   bs_ticket_t* tickets) {
+    bs_ctx_s* c = ctx;
+    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+    if (count < 0 || (count > 0 && (!books || !kinds || !tickets)))
+        return set_error(BS_ERR_INVALID, "bad request arrays");
+    if (count == 0) return BS_OK;
+    int rc;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
+        rc = c->core->submit_many(books, kinds, count, now_ns(), tickets);
+        if (rc == BS_OK) c->changed();
+    }
+    if (rc != BS_OK) return set_error(rc, "bs_submit_many");
+    c->cv_work.notify_one();
+    return BS_OK;
+}
+
+int bs_reap(bs_ctx_t ctx, bs_completion* out, int64_t max, int64_t timeout_us, int64_t* count) {
+    static_assert(sizeof(bs_completion) == sizeof(serve::Completion), "bs_completion layout");
+    bs_ctx_s* c = ctx;
+    if (!c || !count) return set_error(BS_ERR_INVALID, "ctx or count is NULL");
+    *count = 0;
+    if (max < 0 || (max > 0 && !out)) return set_error(BS_ERR_INVALID, "bad output array");
+    auto* dst = reinterpret_cast<serve::Completion*>(out);
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto deadline = t0 + std::chrono::microseconds(timeout_us);
+    std::unique_lock<std::mutex> lk(c->mu);
+    if (!c->core->completions_enabled())
+        return set_error(BS_ERR_INVALID, "context has no completion queue");
+    if (max == 0) return BS_OK;
+    for (;;) {
+        const int64_t seen = c->finished.load(std::memory_order_acquire);
+        const int64_t k = c->core->reap(dst, max);
+        if (k > 0 || timeout_us == 0) {
+            *count = k;
+            return BS_OK;
+        }
+        // Spin for up to kReapSpinUs on the finished counter without the lock
+        // (a batch completes every few hundred microseconds under load, and a
+        // sleeping reaper costs a scheduler wake-up per batch), then sleep.
+        lk.unlock();
+        const auto now = std::chrono::steady_clock::now();
+        const auto spin_end = std::min(now + std::chrono::microseconds(kReapSpinUs),
+                                       timeout_us < 0 ? now + std::chrono::hours(1) : deadline);
+        while (c->finished.load(std::memory_order_acquire) == seen &&
+               std::chrono::steady_clock::now() < spin_end)
+            std::this_thread::yield();
+        lk.lock();
+        if (c->finished.load(std::memory_order_acquire) != seen ||
+            c->core->completions_pending() > 0)
+            continue;
+        auto ready = [&] { return c->core->completions_pending() > 0; };
+        if (timeout_us < 0) {
+            c->cv_done.wait(lk, ready);
+        } else if (!c->cv_done.wait_until(lk, deadline, ready)) {
+            return BS_OK;
+        }
+    }
+}
+
+int bs_result_release_many(bs_ctx_t ctx, const bs_ticket_t* tickets, int64_t count) {
+    bs_ctx_s* c = ctx;
+    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+    if (count < 0 || (count > 0 && !tickets)) return set_error(BS_ERR_INVALID, "bad ticket array");
+    int first = BS_OK;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        for (int64_t i = 0; i < count; ++i) {
+            const int rc = c->core->release_result(tickets[i]);
+            if (rc != BS_OK && first == BS_OK) first = rc;
+        }
+        c->changed();
+    }
+    c->cv_work.notify_all();
+    return first == BS_OK ? BS_OK : set_error(first, "bs_result_release_many");
 }
 
 int bs_poll(bs_ctx_t ctx, bs_ticket_t ticket, int* state) {
@@ -536,6 +675,7 @@ int bs_result_release(bs_ctx_t ctx, bs_ticket_t ticket) {
     {
         std::lock_guard<std::mutex> lk(c->mu);
         rc = c->core->release_result(ticket);
+        if (rc == BS_OK) c->changed();
     }
     if (rc == BS_OK) c->cv_work.notify_all();
     return rc == BS_OK ? BS_OK : set_error(rc, "bs_result_release");
@@ -548,8 +688,15 @@ int bs_cancel(bs_ctx_t ctx, bs_ticket_t ticket) {
     {
         std::lock_guard<std::mutex> lk(c->mu);
         rc = c->core->cancel(ticket, now_ns());
+        if (rc == BS_OK) {
+            c->changed();
+            c->finished.fetch_add(1, std::memory_order_release);
+        }
     }
-    if (rc == BS_OK) c->cv_done.notify_all();
+    if (rc == BS_OK) {
+        c->cv_done.notify_all();
+        c->cv_work.notify_all();   // the cancelled request may have blocked the queue head
+    }
     return rc == BS_OK ? BS_OK : set_error(rc, "bs_cancel: ticket not queued");
 }
 
@@ -581,6 +728,8 @@ int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats* out) {
     out->launches = st.launches;
     out->options_priced = st.options_priced;
     out->admission_waits = st.admission_waits;
+    out->queue_ns_total = st.queue_ns;
+    out->batch_ns_total = c->batch_ns;
     return BS_OK;
 }
 

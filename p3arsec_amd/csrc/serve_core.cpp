@@ -102,7 +102,7 @@ const BookRec* Core::book(int64_t id) const {
     return it == books_.end() ? nullptr : &it->second;
 }
 
-int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket) {
+int Core::check_submit(int64_t book, int kind) const {
     if (kind != kPrice && kind != kGreeks) return kErrInvalid;
     auto it = books_.find(book);
     if (it == books_.end()) return kErrNotFound;
@@ -110,6 +110,42 @@ int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket) {
     // refused now instead of waiting for memory that will never come.
     if (output_bytes(it->second.n, kind) + book_bytes(it->second.n) > arena_.capacity())
         return kErrBudget;
+    return kOk;
+}
+
+int Core::submit_many(const int64_t* books, const int* kinds, int64_t count, int64_t now,
+                      int64_t* tickets) {
+    if (count < 0 || (count > 0 && (!books || !kinds || !tickets))) return kErrInvalid;
+    for (int64_t i = 0; i < count; ++i) {
+        const int rc = check_submit(books[i], kinds[i]);
+        if (rc != kOk) return rc;
+    }
+    for (int64_t i = 0; i < count; ++i) submit(books[i], kinds[i], now, &tickets[i]);
+    return kOk;
+}
+
+int64_t Core::reap(Completion* out, int64_t max) {
+    int64_t k = 0;
+    while (k < max && !completed_.empty()) {
+        const int64_t t = completed_.front();
+        completed_.pop_front();
+        auto it = reqs_.find(t);
+        if (it == reqs_.end()) continue;  // released before it was reaped
+        const Request& r = it->second;
+        Completion& c = out[k++];
+        c.ticket = r.ticket;
+        c.state = r.state;
+        c.status = r.status;
+        c.t_submit = r.t_submit;
+        c.t_done = r.t_done;
+    }
+    return k;
+}
+
+int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket) {
+    const int rc = check_submit(book, kind);
+    if (rc != kOk) return rc;
+    auto it = books_.find(book);
     Request r;
     r.ticket = next_ticket_++;
     r.book = book;
@@ -118,6 +154,7 @@ int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket) {
     r.t_submit = now;
     reqs_[r.ticket] = r;
     pending_.push_back(r.ticket);
+    pending_options_ += r.n;
     ++it->second.refs;
     ++st_.requests_queued;
     *ticket = r.ticket;
@@ -130,31 +167,37 @@ int Core::cancel(int64_t ticket, int64_t now) {
     Request& r = it->second;
     if (r.state != kQueued) return kErrBusy;
     pending_.erase(std::find(pending_.begin(), pending_.end(), ticket));
+    pending_options_ -= r.n;
     r.state = kCancelled;
     r.status = kErrCancelled;
     r.t_done = now;
     books_[r.book].refs--;
     --st_.requests_queued;
     ++st_.requests_cancelled;
+    finished(r);
     return kOk;
 }
 
-std::vector<Request*> Core::admit(int64_t now) {
+std::vector<Request*> Core::admit(int64_t now, int64_t option_cap) {
     std::vector<Request*> batch;
     int64_t options = 0;
+    const int64_t cap =
+        option_cap > 0 ? std::min(option_cap, lim_.max_batch_options) : lim_.max_batch_options;
     while (!pending_.empty() && (int64_t)batch.size() < lim_.max_batch_requests) {
         Request& r = reqs_[pending_.front()];
-        if (!batch.empty() && options + r.n > lim_.max_batch_options) break;
+        if (!batch.empty() && options + r.n > cap) break;
         size_t o;
         if (!arena_.alloc(output_bytes(r.n, r.kind), &o)) {
             ++st_.admission_waits;  // strict FIFO: wait for memory
             break;
         }
         pending_.pop_front();
+        pending_options_ -= r.n;
         r.out_off = o;
         r.has_out = true;
         r.state = kRunning;
         r.t_launch = now;
+        st_.queue_ns += now - r.t_submit;
         options += r.n;
         batch.push_back(&r);
         --st_.requests_queued;
@@ -186,6 +229,7 @@ void Core::complete(const std::vector<int64_t>& tickets, bool ok, int64_t now) {
             r.has_out = false;
             ++st_.requests_failed;
         }
+        finished(r);
     }
 }
 
@@ -193,12 +237,14 @@ void Core::fail_queued(int status, int64_t now) {
     while (!pending_.empty()) {
         Request& r = reqs_[pending_.front()];
         pending_.pop_front();
+        pending_options_ -= r.n;
         r.state = kFailed;
         r.status = status;
         r.t_done = now;
         books_[r.book].refs--;
         --st_.requests_queued;
         ++st_.requests_failed;
+        finished(r);
     }
 }
 

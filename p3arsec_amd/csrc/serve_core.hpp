@@ -76,6 +76,14 @@ struct Request {
     int64_t t_submit = 0, t_launch = 0, t_done = 0;
 };
 
+// One finished request, as the completion queue reports it.
+struct Completion {
+    int64_t ticket = 0;
+    int32_t state = kDone;
+    int32_t status = kOk;
+    int64_t t_submit = 0, t_done = 0;
+};
+
 struct Limits {
     int64_t max_batch_requests = 4096;
     int64_t max_batch_options = int64_t(1) << 30;
@@ -85,6 +93,7 @@ struct Stats {
     int64_t books_resident = 0, requests_queued = 0, requests_running = 0;
     int64_t requests_done = 0, requests_failed = 0, requests_cancelled = 0;
     int64_t launches = 0, options_priced = 0, admission_waits = 0;
+    int64_t queue_ns = 0;  // sum over admitted requests of (admission - submit)
 };
 
 size_t book_bytes(int64_t n);
@@ -104,10 +113,17 @@ class Core {
     const BookRec* book(int64_t id) const;
 
     int submit(int64_t book, int kind, int64_t now, int64_t* ticket);
+    // Submits count requests, all or none: the first invalid one (unknown
+    // book, bad kind, outputs that can never fit) fails the call before any
+    // is queued.
+    int submit_many(const int64_t* books, const int* kinds, int64_t count, int64_t now,
+                    int64_t* tickets);
     int cancel(int64_t ticket, int64_t now);
     // Admits queued requests in FIFO order while their outputs fit in the
     // arena and the batch limits allow; admitted requests become Running.
-    std::vector<Request*> admit(int64_t now);
+    // option_cap > 0 lowers the batch's option limit for this call (the
+    // first request is always admitted if it fits).
+    std::vector<Request*> admit(int64_t now, int64_t option_cap = 0);
     // Marks a launched batch finished.  Failed requests release their outputs.
     void complete(const std::vector<int64_t>& tickets, bool ok, int64_t now);
     // Fails every queued request (shutdown).
@@ -115,7 +131,16 @@ class Core {
     int release_result(int64_t ticket);
     Request* request(int64_t ticket);
 
+    // Completion queue: when enabled, every request that finishes (done,
+    // failed or cancelled) is appended in the order it finished; reap() pops
+    // up to max of them, skipping tickets already released.
+    void enable_completions(bool on) { cq_ = on; }
+    bool completions_enabled() const { return cq_; }
+    int64_t reap(Completion* out, int64_t max);
+    size_t completions_pending() const { return completed_.size(); }
+
     size_t queued() const { return pending_.size(); }
+    int64_t queued_options() const { return pending_options_; }
     const Arena& arena() const { return arena_; }
     Stats stats() const;
 
@@ -126,7 +151,15 @@ class Core {
     std::unordered_map<int64_t, BookRec> books_;
     std::unordered_map<int64_t, Request> reqs_;
     std::deque<int64_t> pending_;
+    std::deque<int64_t> completed_;
+    int64_t pending_options_ = 0;
+    bool cq_ = false;
     Stats st_;
+
+    int check_submit(int64_t book, int kind) const;
+    void finished(const Request& r) {
+        if (cq_) completed_.push_back(r.ticket);
+    }
 };
 
 }  // namespace serve

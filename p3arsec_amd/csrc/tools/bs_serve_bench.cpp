@@ -20,7 +20,6 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
-#include <deque>
 #include <mutex>
 #include <random>
 #include <string>
@@ -49,7 +48,7 @@ void die(const char* what, int rc) {
 
 struct Opts {
     double rate = 20000.0, duration_s = 5.0, greeks_frac = 0.3, budget_gb = 32.0;
-    int books = 256, inflight = 256, use_graph = 1;
+    int books = 256, inflight = 2048, use_graph = 0;
     int64_t max_book = 1 << 20, small = 65536;
     uint64_t seed = 20261207;
     bool saturate = false;
@@ -99,6 +98,7 @@ int main(int argc, char** argv) {
 
     bs_serve_opts so{};
     so.use_graph = o.use_graph;
+    so.completion_queue = 1;
     bs_ctx_t ctx = nullptr;
     CHECK(bs_ctx_create(0, (size_t)(o.budget_gb * (double)(1ull << 30)), &so, &ctx));
 
@@ -127,40 +127,50 @@ int main(int argc, char** argv) {
         catalog_options += n;
     }
 
+    // Completions come from the context's completion queue (bs_reap) in the
+    // order requests finish; each is released right away.  Latency is taken
+    // from the request's arrival time, kept per ticket in a ring indexed by
+    // ticket (tickets of one context are consecutive; the ring is far larger
+    // than the number of outstanding requests).
     std::mutex mu;
-    std::deque<Req> pending;
+    constexpr int64_t kRing = int64_t(1) << 20;
+    std::vector<Req> live((size_t)kRing);
     std::atomic<bool> done_submitting{false};
     std::vector<double> lat_all, lat_small;
+    lat_all.reserve(size_t(1) << 23);
+    lat_small.reserve(size_t(1) << 23);
     int64_t options_done = 0, requests_done = 0;
     std::atomic<int64_t> outstanding{0};
 
     std::thread completer([&] {
+        std::vector<bs_completion> cq(8192);
+        std::vector<bs_ticket_t> rel;
         for (;;) {
-            Req q;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                if (pending.empty()) {
-                    if (done_submitting.load()) break;
-                    q.ticket = -1;
-                } else {
-                    q = pending.front();
-                    pending.pop_front();
-                }
-            }
-            if (q.ticket < 0) {
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            int64_t k = 0;
+            CHECK(bs_reap(ctx, cq.data(), (int64_t)cq.size(), 1000, &k));
+            if (k == 0) {
+                if (done_submitting.load() && outstanding.load() == 0) break;
                 continue;
             }
-            CHECK(bs_wait(ctx, q.ticket, -1));
-            int64_t sub, fin;
-            CHECK(bs_ticket_times(ctx, q.ticket, &sub, &fin));
-            CHECK(bs_result_release(ctx, q.ticket));
-            outstanding.fetch_sub(1);
-            const double ms = (double)(fin - q.arrival) * 1e-6;
-            lat_all.push_back(ms);
-            if (q.n <= o.small) lat_small.push_back(ms);
-            options_done += q.n;
-            ++requests_done;
+            rel.clear();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (int64_t i = 0; i < k; ++i) {
+                    const bs_completion& c = cq[i];
+                    if (c.state != BS_TICKET_DONE) die("request", c.status);
+                    Req& q = live[(size_t)(c.ticket & (kRing - 1))];
+                    if (q.ticket != c.ticket) die("unknown ticket", BS_ERR_NOT_FOUND);
+                    const double ms = (double)(c.done_ns - q.arrival) * 1e-6;
+                    lat_all.push_back(ms);
+                    if (q.n <= o.small) lat_small.push_back(ms);
+                    options_done += q.n;
+                    ++requests_done;
+                    q.ticket = 0;
+                    rel.push_back(c.ticket);
+                }
+            }
+            CHECK(bs_result_release_many(ctx, rel.data(), (int64_t)rel.size()));
+            outstanding.fetch_sub(k);
         }
     });
 
@@ -170,29 +180,53 @@ int main(int argc, char** argv) {
     const int64_t t_end = t0 + (int64_t)(o.duration_s * 1e9);
     double next = (double)t0;
     int64_t submitted = 0;
+    std::vector<bs_book_t> ids;
+    std::vector<int> kinds;
+    std::vector<int64_t> ns;
+    std::vector<bs_ticket_t> tks;
     for (;;) {
+        ids.clear();
+        kinds.clear();
+        ns.clear();
         int64_t arrival;
         if (o.saturate) {
-            while (outstanding.load() >= o.inflight) std::this_thread::yield();
+            // top the window up to --inflight in one call
+            int64_t room;
+            while ((room = o.inflight - outstanding.load()) <= 0) std::this_thread::yield();
             arrival = now_ns();
             if (arrival >= t_end) break;
+            room = std::min<int64_t>(room, 1024);
+            for (int64_t i = 0; i < room; ++i) {
+                const Book& b = books[pick(rng)];
+                ids.push_back(b.id);
+                ns.push_back(b.n);
+                kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
+            }
         } else {
             next += gap(rng) * 1e9;
             arrival = (int64_t)next;
             if (arrival >= t_end) break;
             while (now_ns() < arrival) {
             }
+            const Book& b = books[pick(rng)];
+            ids.push_back(b.id);
+            ns.push_back(b.n);
+            kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
         }
-        const Book& b = books[pick(rng)];
-        const int kind = U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE;
-        bs_ticket_t tk;
-        CHECK(bs_submit(ctx, b.id, kind, &tk));
-        outstanding.fetch_add(1);
+        tks.resize(ids.size());
         {
+            // register before submitting so the completer always finds the ticket
             std::lock_guard<std::mutex> lk(mu);
-            pending.push_back({tk, b.n, arrival, kind});
+            outstanding.fetch_add((int64_t)ids.size());
+            if (ids.size() == 1) CHECK(bs_submit(ctx, ids[0], kinds[0], &tks[0]));
+            else CHECK(bs_submit_many(ctx, ids.data(), kinds.data(), (int64_t)ids.size(), tks.data()));
+            for (size_t i = 0; i < ids.size(); ++i) {
+                Req& q = live[(size_t)(tks[i] & (kRing - 1))];
+                if (q.ticket != 0) die("ticket ring overflow", BS_ERR_INVALID);
+                q = {tks[i], ns[i], arrival, kinds[i]};
+            }
         }
-        ++submitted;
+        submitted += (int64_t)ids.size();
     }
     done_submitting.store(true);
     completer.join();
@@ -206,7 +240,8 @@ int main(int argc, char** argv) {
            "\"p50_ms_le64k\": %.4f, \"p99_ms_le64k\": %.4f, \"max_ms\": %.4f, "
            "\"launches\": %lld, \"mean_requests_per_launch\": %.2f, \"books\": %d, "
            "\"catalog_options\": %lld, \"greeks_frac\": %.2f, \"arena_peak_bytes\": %lld, "
-           "\"admission_waits\": %lld, \"use_graph\": %d}\n",
+           "\"admission_waits\": %lld, \"use_graph\": %d, \"mean_queue_us\": %.2f, "
+           "\"mean_batch_us\": %.2f, \"mean_options_per_launch\": %.0f}\n",
            o.saturate ? "closed-loop saturation" : "open-loop poisson", o.saturate ? 0.0 : o.rate,
            wall, (long long)requests_done, (long long)options_done, (double)options_done / wall,
            pct(lat_all, 0.50), pct(lat_all, 0.99), pct(lat_small, 0.50), pct(lat_small, 0.99),
@@ -214,6 +249,9 @@ int main(int argc, char** argv) {
            (long long)st.launches,
            st.launches ? (double)requests_done / (double)st.launches : 0.0, o.books,
            (long long)catalog_options, o.greeks_frac, (long long)st.arena_peak,
-           (long long)st.admission_waits, o.use_graph);
+           (long long)st.admission_waits, o.use_graph,
+           st.requests_done ? (double)st.queue_ns_total * 1e-3 / (double)st.requests_done : 0.0,
+           st.launches ? (double)st.batch_ns_total * 1e-3 / (double)st.launches : 0.0,
+           st.launches ? (double)st.options_priced / (double)st.launches : 0.0);
     return 0;
 }

@@ -28,6 +28,34 @@ int sc_submit(void* h, int64_t book, int kind, int64_t now, int64_t* ticket) {
     return static_cast<Core*>(h)->submit(book, kind, now, ticket);
 }
 int sc_cancel(void* h, int64_t t, int64_t now) { return static_cast<Core*>(h)->cancel(t, now); }
+int sc_submit_many(void* h, const int64_t* books, const int* kinds, int64_t count, int64_t now,
+                   int64_t* tickets) {
+    return static_cast<Core*>(h)->submit_many(books, kinds, count, now, tickets);
+}
+int sc_admit_cap(void* h, int64_t now, int64_t cap, int64_t* tickets, int cn) {
+    auto b = static_cast<Core*>(h)->admit(now, cap);
+    int i = 0;
+    for (auto* r : b) {
+        if (i < cn) tickets[i] = r->ticket;
+        ++i;
+    }
+    return i;
+}
+int64_t sc_queued_options(void* h) { return static_cast<Core*>(h)->queued_options(); }
+void sc_enable_completions(void* h, int on) { static_cast<Core*>(h)->enable_completions(on != 0); }
+// Pops up to max completions into parallel arrays; returns the count.
+int64_t sc_reap(void* h, int64_t* tickets, int* states, int* statuses, int64_t* done,
+                int64_t max) {
+    std::vector<bs::serve::Completion> v((size_t)(max > 0 ? max : 0));
+    const int64_t k = static_cast<Core*>(h)->reap(v.data(), max);
+    for (int64_t i = 0; i < k; ++i) {
+        tickets[i] = v[i].ticket;
+        states[i] = v[i].state;
+        statuses[i] = v[i].status;
+        done[i] = v[i].t_done;
+    }
+    return k;
+}
 // Admits a batch; writes up to cap tickets and output offsets, returns the count.
 int sc_admit(void* h, int64_t now, int64_t* tickets, int64_t* offs, int cap) {
     auto b = static_cast<Core*>(h)->admit(now);

@@ -4,7 +4,9 @@
   standalone map (bs_price_f32 / bs_price_greeks_f32) -- alone and in a mix;
 * admission by memory: with a small arena, queued requests wait and are
   admitted as results are released; all complete;
-* cancellation, budget refusal, busy books, shutdown with pending work.
+* cancellation, budget refusal, busy books, shutdown with pending work;
+* bulk submission and the completion queue: every request reported once,
+  results identical to the standalone map, memory returned by bulk release.
 """
 import ctypes
 import threading
@@ -20,10 +22,10 @@ i64 = ctypes.c_int64
 
 
 class Ctx:
-    def __init__(self, capi, budget, use_graph=1, max_req=0, grid=0):
+    def __init__(self, capi, budget, use_graph=1, max_req=0, grid=0, cq=0):
         self.capi = capi
         self.L = capi.load()
-        o = capi.ServeOpts(max_req, 0, use_graph, grid)
+        o = capi.ServeOpts(max_req, 0, use_graph, grid, cq, 0)
         h = ctypes.c_void_p()
         capi.call("bs_ctx_create", 0, budget, ctypes.byref(o), ctypes.byref(h))
         self.h = h.value
@@ -190,5 +192,67 @@ def test_resident_book_repriced(gpu):
         s0, s1 = i64(), i64()
         gpu.call("bs_ticket_times", ctx.h, ctx.submit(b, 1), ctypes.byref(s0), ctypes.byref(s1))
         assert s0.value > 0
+    finally:
+        ctx.destroy()
+
+
+def test_submit_many_and_completion_queue(gpu):
+    rng = np.random.default_rng(11)
+    sizes = [1, 7, 1024, 5000, 65_537, 200_000]
+    bks = [books.make_book(n, books.SEED_BASE + 70 + i) for i, n in enumerate(sizes)]
+    ref = {(i, k): standalone(gpu, bks[i], k) for i in range(len(bks)) for k in (1, 2)}
+    # the arena holds the books and about 6 Greeks outputs of the largest book,
+    # so bulk submissions wait for memory that only bulk releases return
+    book_bytes = sum(5 * (((n * 4 + 255) // 256) * 256) + ((n + 255) // 256) * 256
+                     for n in sizes)
+    budget = book_bytes + 6 * 5 * ((200_000 * 4 + 255) // 256) * 256
+    ctx = Ctx(gpu, budget, cq=1)
+    L = ctx.L
+    try:
+        ids = [ctx.register(b) for b in bks]
+        reqs = [(int(rng.integers(len(bks))), int(rng.integers(1, 3))) for _ in range(300)]
+        kinds = {}
+        for lo in range(0, len(reqs), 100):
+            chunk = reqs[lo:lo + 100]
+            bk = (i64 * len(chunk))(*[ids[i] for i, _ in chunk])
+            kd = (ctypes.c_int * len(chunk))(*[k for _, k in chunk])
+            tk = (i64 * len(chunk))()
+            gpu.call("bs_submit_many", ctx.h, bk, kd, len(chunk), tk)
+            kinds.update({t: r for t, r in zip(tk, chunk)})
+        cq = (gpu.Completion * 64)()
+        cnt = i64()
+        seen = []
+        deadline = time.time() + 60
+        while len(seen) < len(reqs) and time.time() < deadline:
+            gpu.call("bs_reap", ctx.h, cq, 64, 100_000, ctypes.byref(cnt))
+            got = [cq[j] for j in range(cnt.value)]
+            for c in got:
+                assert c.state == gpu.TICKET_DONE and c.status == 0
+                assert c.done_ns >= c.submit_ns > 0
+                i, k = kinds[c.ticket]
+                assert same_bits(ctx.result(c.ticket, bks[i].n, k), ref[(i, k)]), (i, k)
+            rel = (i64 * max(1, len(got)))(*[c.ticket for c in got])
+            gpu.call("bs_result_release_many", ctx.h, rel, len(got))
+            seen += [c.ticket for c in got]
+        assert sorted(seen) == sorted(kinds)          # each request reported once
+        st = ctx.stats()
+        assert st.requests_done == len(reqs) and st.admission_waits > 0
+        assert st.arena_used == book_bytes            # every output returned
+        # errors: all-or-nothing submit, double release, reap with max 0
+        bk = (i64 * 2)(ids[0], 424242)
+        kd = (ctypes.c_int * 2)(1, 1)
+        assert L.bs_submit_many(ctx.h, bk, kd, 2, (i64 * 2)()) == -5
+        gpu.call("bs_reap", ctx.h, cq, 64, 0, ctypes.byref(cnt))
+        assert cnt.value == 0                         # the failed call queued nothing
+        assert L.bs_result_release_many(ctx.h, (i64 * 1)(seen[0]), 1) == -5
+    finally:
+        ctx.destroy()
+
+
+def test_reap_needs_completion_queue(gpu):
+    ctx = Ctx(gpu, 16 << 20)
+    try:
+        cnt = i64()
+        assert ctx.L.bs_reap(ctx.h, (gpu.Completion * 4)(), 4, 0, ctypes.byref(cnt)) == -1
     finally:
         ctx.destroy()

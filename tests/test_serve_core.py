@@ -32,6 +32,11 @@ def L():
         "sc_release_book": ([vp, i64], ctypes.c_int),
         "sc_submit": ([vp, i64, ctypes.c_int, i64, P(i64)], ctypes.c_int),
         "sc_cancel": ([vp, i64, i64], ctypes.c_int),
+        "sc_submit_many": ([vp, P(i64), P(ctypes.c_int), i64, i64, P(i64)], ctypes.c_int),
+        "sc_enable_completions": ([vp, ctypes.c_int], None),
+        "sc_admit_cap": ([vp, i64, i64, P(i64), ctypes.c_int], ctypes.c_int),
+        "sc_queued_options": ([vp], i64),
+        "sc_reap": ([vp, P(i64), P(ctypes.c_int), P(ctypes.c_int), P(i64), i64], i64),
         "sc_admit": ([vp, i64, P(i64), P(i64), ctypes.c_int], ctypes.c_int),
         "sc_complete": ([vp, P(i64), ctypes.c_int, ctypes.c_int, i64], None),
         "sc_fail_queued": ([vp, ctypes.c_int, i64], None),
@@ -240,3 +245,89 @@ def test_never_fitting_request_is_refused(L):
     assert rc == OK
     assert c.submit(b, PRICE)[0] == OK
     assert c.submit(b, GREEKS)[0] == BUDGET
+
+
+def reap(c, cap=64):
+    t, s, st, d = (i64 * cap)(), (ctypes.c_int * cap)(), (ctypes.c_int * cap)(), (i64 * cap)()
+    k = c.L.sc_reap(c.h, t, s, st, d, cap)
+    return [(t[i], s[i], st[i]) for i in range(k)]
+
+
+def test_completion_queue_order_and_states(L):
+    """Every finished request (done, failed, cancelled, shut down) is reported
+    once, in the order it finished; released tickets are skipped."""
+    c = Core(L, 1 << 22)
+    L.sc_enable_completions(c.h, 1)
+    _, b, _ = c.book(100)
+    t = [c.submit(b)[1] for _ in range(6)]
+    assert L.sc_cancel(c.h, t[1], 0) == OK
+    got, _ = c.admit(cap=2)
+    assert got == [t[0], t[2], t[3], t[4], t[5]][:len(got)]
+    c.complete([t[2], t[0]])                           # completion order, not ticket order
+    c.complete([x for x in got if x not in (t[0], t[2])], ok=False)
+    assert reap(c, 1) == [(t[1], CANCELLED_ST, CANCELLED)]
+    rest = reap(c)
+    assert rest[:2] == [(t[2], DONE, OK), (t[0], DONE, OK)]
+    assert all(s == FAILED_ST for _, s, _ in rest[2:])
+    assert reap(c) == []
+    # a ticket released before it is reaped is skipped
+    _, t7 = c.submit(b)
+    c.admit()
+    c.complete([t7])
+    assert L.sc_release_result(c.h, t7) == OK
+    assert reap(c) == []
+    _, t8 = c.submit(b)
+    L.sc_fail_queued(c.h, SHUTDOWN, 5)
+    assert reap(c) == [(t8, FAILED_ST, SHUTDOWN)]
+
+
+def test_completion_queue_off_by_default(L):
+    c = Core(L, 1 << 20)
+    _, b, _ = c.book(10)
+    _, t = c.submit(b)
+    c.admit()
+    c.complete([t])
+    assert reap(c) == []
+
+
+def test_submit_many_is_all_or_nothing(L):
+    c = Core(L, 1 << 22)
+    _, b1, _ = c.book(100)
+    _, b2, _ = c.book(50)
+    books = (i64 * 4)(b1, b2, b1, b2)
+    kinds = (ctypes.c_int * 4)(PRICE, GREEKS, GREEKS, PRICE)
+    tk = (i64 * 4)()
+    assert L.sc_submit_many(c.h, books, kinds, 4, 1, tk) == OK
+    assert list(tk) == sorted(tk) and L.sc_queued(c.h) == 4
+    got, _ = c.admit()
+    assert got == list(tk)                             # queued in call order
+    bad_book = (i64 * 3)(b1, 999, b2)
+    assert L.sc_submit_many(c.h, bad_book, kinds, 3, 2, tk) == NOT_FOUND
+    bad_kind = (ctypes.c_int * 3)(PRICE, 7, PRICE)
+    assert L.sc_submit_many(c.h, books, bad_kind, 3, 2, tk) == INVALID
+    assert L.sc_queued(c.h) == 0                       # nothing of a failed call queued
+    assert L.sc_submit_many(c.h, books, kinds, 0, 2, tk) == OK
+
+
+def test_admit_option_cap_and_queued_options(L):
+    """A per-call option cap splits a long queue; the first request is
+    admitted even when it alone exceeds the cap.  queued_options tracks the
+    queue through submit, admit, cancel and shutdown."""
+    c = Core(L, 1 << 24)
+    _, b, _ = c.book(1000)
+    _, big, _ = c.book(5000)
+    ts = [c.submit(b)[1] for _ in range(6)]
+    assert L.sc_queued_options(c.h) == 6000
+    tk = (i64 * 16)()
+    assert L.sc_admit_cap(c.h, 1, 3000, tk, 16) == 3 and list(tk[:3]) == ts[:3]
+    assert L.sc_queued_options(c.h) == 3000
+    _, tb = c.submit(big)
+    assert L.sc_cancel(c.h, ts[4], 0) == OK
+    assert L.sc_queued_options(c.h) == 7000
+    assert L.sc_admit_cap(c.h, 2, 1500, tk, 16) == 1 and tk[0] == ts[3]
+    assert L.sc_admit_cap(c.h, 3, 1500, tk, 16) == 1 and tk[0] == ts[5]
+    assert L.sc_admit_cap(c.h, 4, 1500, tk, 16) == 1 and tk[0] == tb   # over the cap alone
+    assert L.sc_queued_options(c.h) == 0
+    c.submit(b)
+    L.sc_fail_queued(c.h, SHUTDOWN, 9)
+    assert L.sc_queued_options(c.h) == 0
