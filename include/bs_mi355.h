@@ -198,6 +198,8 @@ typedef struct {
  * (CLOCK_MONOTONIC) nanoseconds. */
 typedef struct {
     bs_ticket_t ticket;
+    int64_t user;   /* the caller's tag from bs_submit_many (0 otherwise)      */
+    int64_t n;      /* options of the request's book                         */
     int32_t state;  /* BS_TICKET_DONE, BS_TICKET_FAILED or BS_TICKET_CANCELLED */
     int32_t status; /* BS_OK or the request's error code                     */
     int64_t submit_ns, done_ns;
@@ -210,6 +212,11 @@ typedef struct {
     int64_t launches, options_priced, admission_waits;
     int64_t queue_ns_total; /* sum over admitted requests of submit -> launch  */
     int64_t batch_ns_total; /* sum over batches of launch -> completion seen   */
+    /* dispatcher thread: time admitting (lock wait + admission + table),
+     * launching, retiring; time with no batch in flight and the launches
+     * that ended such a stretch */
+    int64_t dispatch_admit_ns, dispatch_launch_ns, dispatch_retire_ns;
+    int64_t gpu_idle_ns, idle_launches;
 } bs_serve_stats;
 
 int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts *opts,
@@ -225,9 +232,10 @@ int bs_book_release(bs_ctx_t ctx, bs_book_t book);
 int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t *ticket);
 /* Submits count requests under one lock, all or none: an unknown book, a bad
  * kind or outputs that can never fit fail the call before any is queued.
- * Requests of one call join the queue in order. */
-int bs_submit_many(bs_ctx_t ctx, const bs_book_t *books, const int *kinds, int64_t count,
-                   bs_ticket_t *tickets);
+ * Requests of one call join the queue in order.  users (may be NULL) gives
+ * each request a tag that its bs_completion carries back. */
+int bs_submit_many(bs_ctx_t ctx, const bs_book_t *books, const int *kinds,
+                   const int64_t *users, int64_t count, bs_ticket_t *tickets);
 /* With opts.completion_queue set: pops up to max finished requests in the
  * order they finished, waiting up to timeout_us for the first one (0 polls,
  * < 0 waits forever).  *count = 0 after a timeout.  Outputs stay resident

@@ -53,8 +53,8 @@ class ServeOpts(ctypes.Structure):
 
 
 class Completion(ctypes.Structure):
-    _fields_ = [("ticket", _i64), ("state", ctypes.c_int32), ("status", ctypes.c_int32),
-                ("submit_ns", _i64), ("done_ns", _i64)]
+    _fields_ = [("ticket", _i64), ("user", _i64), ("n", _i64), ("state", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("submit_ns", _i64), ("done_ns", _i64)]
 
 
 class ServeStats(ctypes.Structure):
@@ -62,7 +62,8 @@ class ServeStats(ctypes.Structure):
         "arena_bytes", "arena_used", "arena_peak", "books_resident", "requests_queued",
         "requests_running", "requests_done", "requests_failed", "requests_cancelled",
         "launches", "options_priced", "admission_waits", "queue_ns_total",
-        "batch_ns_total")]
+        "batch_ns_total", "dispatch_admit_ns", "dispatch_launch_ns", "dispatch_retire_ns",
+        "gpu_idle_ns", "idle_launches")]
 
 
 # name -> (argtypes, restype); restype int unless given
@@ -107,8 +108,8 @@ _SIGS = {
     "bs_book_register": ([_vp] + [_vp] * 6 + [_i64, ctypes.POINTER(_i64)], _i32),
     "bs_book_release": ([_vp, _i64], _i32),
     "bs_submit": ([_vp, _i64, _i32, ctypes.POINTER(_i64)], _i32),
-    "bs_submit_many": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int), _i64,
-                        ctypes.POINTER(_i64)], _i32),
+    "bs_submit_many": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
+                        ctypes.POINTER(_i64), _i64, ctypes.POINTER(_i64)], _i32),
     "bs_reap": ([_vp, ctypes.POINTER(Completion), _i64, _i64, ctypes.POINTER(_i64)], _i32),
     "bs_result_release_many": ([_vp, ctypes.POINTER(_i64), _i64], _i32),
     "bs_poll": ([_vp, _i64, ctypes.POINTER(_i32)], _i32),

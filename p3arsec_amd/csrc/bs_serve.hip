@@ -19,6 +19,7 @@
 //     completes.  Requests that arrive while a batch runs go into the next one.
 // Results are bit-identical to bs_price_f32 / bs_price_greeks_f32: the fused
 // kernels run the batch map's own per-group code (bs_ops.hpp).
+#include <stddef.h>
 #include <string.h>
 #include <time.h>
 
@@ -186,6 +187,8 @@ struct bs_ctx_s {
     std::atomic<int64_t> gen{0};
     std::atomic<int64_t> queued{0};
     std::atomic<int64_t> finished{0};  // requests finished so far (done, failed, cancelled)
+    // Dispatcher time accounting (written by the dispatcher only).
+    std::atomic<int64_t> t_admit{0}, t_launch{0}, t_retire{0}, t_idle{0}, idle_launches{0};
     std::atomic<bool> shutdown{false};
     std::thread worker;
     std::string fatal;
@@ -245,29 +248,29 @@ hipError_t capture_slot(bs_ctx_s* c, int s) {
 }
 
 // Fills slot s with the batch's descriptors, price requests first, each
-// section in admission order (caller holds the lock).
-void write_slot(bs_ctx_s* c, int s, const std::vector<serve::Request*>& reqs) {
+// section in admission order.  Runs without the lock: it reads only the
+// admitted requests' copies, and the slot is the dispatcher's own.
+void write_slot(bs_ctx_s* c, int s, const std::vector<serve::Admitted>& reqs) {
     SlotHeader* h = reinterpret_cast<SlotHeader*>(c->host_slot[s]);
     Desc* d = reinterpret_cast<Desc*>(c->host_slot[s] + sizeof(SlotHeader));
     int nd[2] = {0, 0};
-    for (const serve::Request* r : reqs) ++nd[r->kind == serve::kGreeks ? 1 : 0];
+    for (const serve::Admitted& r : reqs) ++nd[r.kind == serve::kGreeks ? 1 : 0];
     int next[2] = {0, nd[0]};
     int64_t tiles[2] = {0, 0};
-    for (const serve::Request* r : reqs) {
-        const int sec = r->kind == serve::kGreeks ? 1 : 0;
-        const serve::BookRec* b = c->core->book(r->book);
+    for (const serve::Admitted& r : reqs) {
+        const int sec = r.kind == serve::kGreeks ? 1 : 0;
         size_t bo[6], oo[5];
-        serve::book_layout(b->n, bo);
-        serve::output_layout(r->n, r->kind, oo);
+        serve::book_layout(r.book_n, bo);
+        serve::output_layout(r.n, r.kind, oo);
         Desc& x = d[next[sec]++];
-        for (int j = 0; j < 5; ++j) x.in[j] = reinterpret_cast<const float*>(c->arena + b->off + bo[j]);
-        x.type = reinterpret_cast<const uint8_t*>(c->arena + b->off + bo[5]);
-        for (int j = 0; j < 5; ++j) x.out[j] = reinterpret_cast<float*>(c->arena + r->out_off + oo[j]);
-        x.n = r->n;
+        for (int j = 0; j < 5; ++j) x.in[j] = reinterpret_cast<const float*>(c->arena + r.book_off + bo[j]);
+        x.type = reinterpret_cast<const uint8_t*>(c->arena + r.book_off + bo[5]);
+        for (int j = 0; j < 5; ++j) x.out[j] = reinterpret_cast<float*>(c->arena + r.out_off + oo[j]);
+        x.n = r.n;
         x.tile_begin = tiles[sec];
-        x.kind = r->kind;
+        x.kind = r.kind;
         x.pad = 0;
-        tiles[sec] += (r->n + kTile - 1) / kTile;
+        tiles[sec] += (r.n + kTile - 1) / kTile;
     }
     h->ndesc[0] = nd[0];
     h->ndesc[1] = nd[1];
@@ -281,6 +284,9 @@ void dispatcher(bs_ctx_s* c) {
     std::vector<Batch> inflight;        // launch order; owned by this thread
     bool busy[kSlots] = {};
     int64_t blocked_gen = -1;           // gen at the last admission that found no room
+    int64_t t_empty = -1;               // when the last batch in flight retired
+    std::vector<serve::Admitted> reqs;  // reused: no per-batch allocation
+    reqs.reserve(kMaxDesc);
     for (;;) {
         bool progressed = false;
         // retire finished batches in launch order
@@ -300,6 +306,9 @@ void dispatcher(bs_ctx_s* c) {
             c->cv_done.notify_all();
             busy[b.slot] = false;
             inflight.erase(inflight.begin());
+            const int64_t t2 = now_ns();
+            c->t_retire.fetch_add(t2 - t, std::memory_order_relaxed);
+            if (inflight.empty()) t_empty = t2;
             blocked_gen = -1;   // a failed batch frees its outputs
             progressed = true;
         }
@@ -310,25 +319,29 @@ void dispatcher(bs_ctx_s* c) {
         if (s >= 0 && !c->shutdown.load(std::memory_order_acquire) &&
             c->queued.load(std::memory_order_acquire) > 0 &&
             c->gen.load(std::memory_order_acquire) != blocked_gen) {
-            std::vector<serve::Request*> reqs;
             Batch b{s, c->ev[s], 0, {}};
+            const int64_t ta = now_ns();
             {
                 std::lock_guard<std::mutex> lk(c->mu);
                 const int64_t g = c->gen.load(std::memory_order_relaxed);
                 const int64_t qo = c->core->queued_options();
-                reqs = c->core->admit(now_ns(), inflight.size() < 2 && qo > kSplitOptions ? qo / 2 : 0);
-                if (reqs.empty()) {
-                    blocked_gen = g;   // the queue head does not fit: wait for a change
-                } else {
-                    write_slot(c, s, reqs);
-                    for (auto* r : reqs) b.tickets.push_back(r->ticket);
-                    c->changed();
-                }
+                c->core->admit(now_ns(), &reqs, inflight.size() < 2 && qo > kSplitOptions ? qo / 2 : 0);
+                if (reqs.empty()) blocked_gen = g;   // the queue head does not fit: wait for a change
+                else c->changed();
             }
             if (!reqs.empty()) {
+                write_slot(c, s, reqs);
+                b.tickets.reserve(reqs.size());
+                for (const auto& r : reqs) b.tickets.push_back(r.ticket);
                 b.t_launch = now_ns();
+                c->t_admit.fetch_add(b.t_launch - ta, std::memory_order_relaxed);
+                if (inflight.empty() && t_empty >= 0) {
+                    c->t_idle.fetch_add(b.t_launch - t_empty, std::memory_order_relaxed);
+                    c->idle_launches.fetch_add(1, std::memory_order_relaxed);
+                }
                 hipError_t e = launch_slot(c, s);
                 if (e == hipSuccess) e = hipEventRecord(c->ev[s], c->stream);
+                c->t_launch.fetch_add(now_ns() - b.t_launch, std::memory_order_relaxed);
                 if (e != hipSuccess) {
                     {
                         std::lock_guard<std::mutex> lk(c->mu);
@@ -511,8 +524,8 @@ int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t* ticket) {
     return BS_OK;
 }
 
-int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, int64_t count,
-                   bs_ticket_t* tickets) {
+int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, const int64_t* users,
+                   int64_t count, bs_ticket_t* tickets) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
     if (count < 0 || (count > 0 && (!books || !kinds || !tickets)))
@@ -522,7 +535,7 @@ int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, int64
     {
         std::lock_guard<std::mutex> lk(c->mu);
         if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
-        rc = c->core->submit_many(books, kinds, count, now_ns(), tickets);
+        rc = c->core->submit_many(books, kinds, users, count, now_ns(), tickets);
         if (rc == BS_OK) c->changed();
     }
     if (rc != BS_OK) return set_error(rc, "bs_submit_many");
@@ -531,7 +544,11 @@ int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, int64
 }
 
 int bs_reap(bs_ctx_t ctx, bs_completion* out, int64_t max, int64_t timeout_us, int64_t* count) {
-    static_assert(sizeof(bs_completion) == sizeof(serve::Completion), "bs_completion layout");
+    static_assert(sizeof(bs_completion) == sizeof(serve::Completion) &&
+                      offsetof(bs_completion, user) == offsetof(serve::Completion, user) &&
+                      offsetof(bs_completion, state) == offsetof(serve::Completion, state) &&
+                      offsetof(bs_completion, done_ns) == offsetof(serve::Completion, t_done),
+                  "bs_completion layout");
     bs_ctx_s* c = ctx;
     if (!c || !count) return set_error(BS_ERR_INVALID, "ctx or count is NULL");
     *count = 0;
@@ -730,6 +747,11 @@ int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats* out) {
     out->admission_waits = st.admission_waits;
     out->queue_ns_total = st.queue_ns;
     out->batch_ns_total = c->batch_ns;
+    out->dispatch_admit_ns = c->t_admit.load();
+    out->dispatch_launch_ns = c->t_launch.load();
+    out->dispatch_retire_ns = c->t_retire.load();
+    out->gpu_idle_ns = c->t_idle.load();
+    out->idle_launches = c->idle_launches.load();
     return BS_OK;
 }
 

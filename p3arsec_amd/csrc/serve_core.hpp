@@ -4,7 +4,9 @@
 // against them.  This header holds everything that is bookkeeping: the arena
 // allocator, the book and request tables, FIFO admission by memory and batch
 // formation.  It has no GPU dependency so it is unit-tested on the CPU; the
-// HIP executor (bs_serve.hip) drives it under the context's mutex.
+// HIP executor (bs_serve.hip) drives it under the context's mutex, so every
+// operation here is O(1) per request (no scans, no per-request heap
+// allocation in steady state) to keep that mutex's hold times short.
 //
 // Arena layout (offsets from the arena base, every block 256-byte aligned):
 //   book:    s | k | r | v | t (float, n each) | type (uint8, n)
@@ -15,7 +17,7 @@
 #include <stdint.h>
 
 #include <deque>
-#include <map>
+#include <memory>
 #include <unordered_map>
 #include <vector>
 
@@ -25,21 +27,41 @@ namespace serve {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
-// First-fit allocator over [0, capacity) with coalescing of free neighbours.
+// Two-level segregated-fit allocator (TLSF, Masmano et al., ECRTS 2004) over
+// [0, capacity) in kAlign units.  Free blocks sit in per-size-class lists
+// indexed by two bitmaps: 16 classes per power of two.  alloc() takes the
+// first non-empty class whose every block fits (O(1)), else searches the
+// request's own class (so it fails only when no free block is large enough);
+// free() coalesces with both address neighbours in O(1).  Block records live
+// in host memory (the arena itself is device memory) and are named by handle.
 class Arena {
    public:
     explicit Arena(size_t capacity);
-    bool alloc(size_t bytes, size_t* off);
-    bool free(size_t off);  // false if off is not a live block
+    // Returns a handle >= 0 and the block's offset, or -1 when nothing fits.
+    int64_t alloc(size_t bytes, size_t* off);
+    bool free(int64_t handle);  // false if handle is not a live block
     size_t capacity() const { return capacity_; }
     size_t used() const { return used_; }
     size_t peak() const { return peak_; }
     size_t largest_free() const;
 
    private:
+    static constexpr int kSL = 16, kFL = 48;
+    struct Block {
+        uint64_t off = 0, units = 0;
+        int32_t prev_phys = -1, next_phys = -1, prev_free = -1, next_free = -1;
+        bool free = false, live = false;
+    };
+    static void mapping(uint64_t units, int* fl, int* sl);
+    int32_t new_record();
+    void insert_free(int32_t b);
+    void remove_free(int32_t b);
+    std::vector<Block> blk_;
+    std::vector<int32_t> spare_;  // unused record indices
+    int32_t head_[kFL][kSL];
+    uint64_t fl_map_ = 0;
+    uint32_t sl_map_[kFL] = {};
     size_t capacity_, used_ = 0, peak_ = 0;
-    std::map<size_t, size_t> free_;            // offset -> size
-    std::unordered_map<size_t, size_t> live_;  // offset -> size
 };
 
 // Status and state codes match include/bs_mi355.h.
@@ -61,24 +83,40 @@ struct BookRec {
     int64_t id = 0;
     int64_t n = 0;
     size_t off = 0;
+    int64_t blk = -1;
     int refs = 0;  // queued + running requests
 };
 
 struct Request {
-    int64_t ticket = 0;
+    int64_t ticket = 0;  // 0: slot unused
     int64_t book = 0;
+    BookRec* bookp = nullptr;  // stable: a referenced book cannot be released
+    int64_t user = 0;
     int kind = kPrice;
     int64_t n = 0;
     int state = kQueued;
     int status = kOk;
     size_t out_off = 0;
-    bool has_out = false;
+    int64_t out_blk = -1;
     int64_t t_submit = 0, t_launch = 0, t_done = 0;
 };
 
-// One finished request, as the completion queue reports it.
+// What the executor needs of an admitted request to build its descriptor;
+// a copy, so the table can be written after the lock is released.
+struct Admitted {
+    int64_t ticket;
+    int64_t n;
+    size_t book_off, out_off;
+    int64_t book_n;
+    int kind;
+};
+
+// One finished request, as the completion queue reports it (the layout of
+// bs_completion).
 struct Completion {
     int64_t ticket = 0;
+    int64_t user = 0;
+    int64_t n = 0;
     int32_t state = kDone;
     int32_t status = kOk;
     int64_t t_submit = 0, t_done = 0;
@@ -112,18 +150,19 @@ class Core {
     int release_book(int64_t id);
     const BookRec* book(int64_t id) const;
 
-    int submit(int64_t book, int kind, int64_t now, int64_t* ticket);
+    int submit(int64_t book, int kind, int64_t now, int64_t* ticket, int64_t user = 0);
     // Submits count requests, all or none: the first invalid one (unknown
     // book, bad kind, outputs that can never fit) fails the call before any
-    // is queued.
-    int submit_many(const int64_t* books, const int* kinds, int64_t count, int64_t now,
-                    int64_t* tickets);
+    // is queued.  users may be NULL (tags 0).
+    int submit_many(const int64_t* books, const int* kinds, const int64_t* users, int64_t count,
+                    int64_t now, int64_t* tickets);
     int cancel(int64_t ticket, int64_t now);
     // Admits queued requests in FIFO order while their outputs fit in the
-    // arena and the batch limits allow; admitted requests become Running.
-    // option_cap > 0 lowers the batch's option limit for this call (the
-    // first request is always admitted if it fits).
-    std::vector<Request*> admit(int64_t now, int64_t option_cap = 0);
+    // arena and the batch limits allow; admitted requests become Running and
+    // are appended to *out (cleared first).  option_cap > 0 lowers the
+    // batch's option limit for this call (the first request is always
+    // admitted if it fits).  Returns the number admitted.
+    size_t admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap = 0);
     // Marks a launched batch finished.  Failed requests release their outputs.
     void complete(const std::vector<int64_t>& tickets, bool ok, int64_t now);
     // Fails every queued request (shutdown).
@@ -145,21 +184,30 @@ class Core {
     Stats stats() const;
 
    private:
+    // Tickets are (serial << kSlotBits) | slot: O(1) lookup, and a ticket
+    // whose slot was reused no longer matches.
+    static constexpr int kSlotBits = 24;
+    static constexpr int kChunk = 4096;
+    Request* slot_of(int64_t ticket);
+    int64_t new_ticket(Request** r);
+    void free_slot(Request* r);
+    int check_submit(int64_t book, int kind);
+    void finished(const Request& r) {
+        if (cq_) completed_.push_back(r.ticket);
+    }
+
     Arena arena_;
     Limits lim_;
-    int64_t next_book_ = 1, next_ticket_ = 1;
+    int64_t next_book_ = 1, next_serial_ = 1;
     std::unordered_map<int64_t, BookRec> books_;
-    std::unordered_map<int64_t, Request> reqs_;
+    std::vector<std::unique_ptr<Request[]>> chunks_;  // stable request storage
+    std::vector<int32_t> spare_slots_;
+    int64_t nslots_ = 0;
     std::deque<int64_t> pending_;
     std::deque<int64_t> completed_;
     int64_t pending_options_ = 0;
     bool cq_ = false;
     Stats st_;
-
-    int check_submit(int64_t book, int kind) const;
-    void finished(const Request& r) {
-        if (cq_) completed_.push_back(r.ticket);
-    }
 };
 
 }  // namespace serve

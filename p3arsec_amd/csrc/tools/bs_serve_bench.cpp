@@ -20,7 +20,6 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
-#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
@@ -56,11 +55,6 @@ struct Opts {
 
 struct Book {
     int64_t id, n;
-};
-
-struct Req {
-    int64_t ticket, n, arrival;
-    int kind;
 };
 
 double pct(std::vector<double>& v, double p) {
@@ -128,49 +122,44 @@ int main(int argc, char** argv) {
     }
 
     // Completions come from the context's completion queue (bs_reap) in the
-    // order requests finish; each is released right away.  Latency is taken
-    // from the request's arrival time, kept per ticket in a ring indexed by
-    // ticket (tickets of one context are consecutive; the ring is far larger
-    // than the number of outstanding requests).
-    std::mutex mu;
-    constexpr int64_t kRing = int64_t(1) << 20;
-    std::vector<Req> live((size_t)kRing);
+    // order requests finish; each is released right away.  Each request's tag
+    // is its arrival time and its completion carries its size, so the two
+    // client threads share nothing but the outstanding count.
     std::atomic<bool> done_submitting{false};
     std::vector<double> lat_all, lat_small;
     lat_all.reserve(size_t(1) << 23);
     lat_small.reserve(size_t(1) << 23);
     int64_t options_done = 0, requests_done = 0;
     std::atomic<int64_t> outstanding{0};
+    int64_t c_wait = 0, c_work = 0, s_wait = 0, s_work = 0;   // client thread time, ns
 
     std::thread completer([&] {
         std::vector<bs_completion> cq(8192);
         std::vector<bs_ticket_t> rel;
         for (;;) {
             int64_t k = 0;
+            const int64_t w0 = now_ns();
             CHECK(bs_reap(ctx, cq.data(), (int64_t)cq.size(), 1000, &k));
+            const int64_t w1 = now_ns();
+            c_wait += w1 - w0;
             if (k == 0) {
                 if (done_submitting.load() && outstanding.load() == 0) break;
                 continue;
             }
             rel.clear();
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                for (int64_t i = 0; i < k; ++i) {
-                    const bs_completion& c = cq[i];
-                    if (c.state != BS_TICKET_DONE) die("request", c.status);
-                    Req& q = live[(size_t)(c.ticket & (kRing - 1))];
-                    if (q.ticket != c.ticket) die("unknown ticket", BS_ERR_NOT_FOUND);
-                    const double ms = (double)(c.done_ns - q.arrival) * 1e-6;
-                    lat_all.push_back(ms);
-                    if (q.n <= o.small) lat_small.push_back(ms);
-                    options_done += q.n;
-                    ++requests_done;
-                    q.ticket = 0;
-                    rel.push_back(c.ticket);
-                }
+            for (int64_t i = 0; i < k; ++i) {
+                const bs_completion& c = cq[i];
+                if (c.state != BS_TICKET_DONE) die("request", c.status);
+                const double ms = (double)(c.done_ns - c.user) * 1e-6;
+                lat_all.push_back(ms);
+                if (c.n <= o.small) lat_small.push_back(ms);
+                options_done += c.n;
+                ++requests_done;
+                rel.push_back(c.ticket);
             }
             CHECK(bs_result_release_many(ctx, rel.data(), (int64_t)rel.size()));
             outstanding.fetch_sub(k);
+            c_work += now_ns() - w1;
         }
     });
 
@@ -182,51 +171,40 @@ int main(int argc, char** argv) {
     int64_t submitted = 0;
     std::vector<bs_book_t> ids;
     std::vector<int> kinds;
-    std::vector<int64_t> ns;
+    std::vector<int64_t> tags;
     std::vector<bs_ticket_t> tks;
     for (;;) {
         ids.clear();
         kinds.clear();
-        ns.clear();
+        tags.clear();
         int64_t arrival;
+        int64_t want = 1;
         if (o.saturate) {
             // top the window up to --inflight in one call
             int64_t room;
+            const int64_t w0 = now_ns();
             while ((room = o.inflight - outstanding.load()) <= 0) std::this_thread::yield();
             arrival = now_ns();
+            s_wait += arrival - w0;
             if (arrival >= t_end) break;
-            room = std::min<int64_t>(room, 1024);
-            for (int64_t i = 0; i < room; ++i) {
-                const Book& b = books[pick(rng)];
-                ids.push_back(b.id);
-                ns.push_back(b.n);
-                kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
-            }
+            want = std::min<int64_t>(room, 1024);
         } else {
             next += gap(rng) * 1e9;
-            arrival = (int64_t)next;
+            arrival = (int64_t)next;   // latency counts from the scheduled arrival
             if (arrival >= t_end) break;
             while (now_ns() < arrival) {
             }
-            const Book& b = books[pick(rng)];
-            ids.push_back(b.id);
-            ns.push_back(b.n);
+        }
+        for (int64_t i = 0; i < want; ++i) {
+            ids.push_back(books[pick(rng)].id);
             kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
+            tags.push_back(arrival);
         }
         tks.resize(ids.size());
-        {
-            // register before submitting so the completer always finds the ticket
-            std::lock_guard<std::mutex> lk(mu);
-            outstanding.fetch_add((int64_t)ids.size());
-            if (ids.size() == 1) CHECK(bs_submit(ctx, ids[0], kinds[0], &tks[0]));
-            else CHECK(bs_submit_many(ctx, ids.data(), kinds.data(), (int64_t)ids.size(), tks.data()));
-            for (size_t i = 0; i < ids.size(); ++i) {
-                Req& q = live[(size_t)(tks[i] & (kRing - 1))];
-                if (q.ticket != 0) die("ticket ring overflow", BS_ERR_INVALID);
-                q = {tks[i], ns[i], arrival, kinds[i]};
-            }
-        }
-        submitted += (int64_t)ids.size();
+        outstanding.fetch_add(want);
+        CHECK(bs_submit_many(ctx, ids.data(), kinds.data(), tags.data(), want, tks.data()));
+        submitted += want;
+        s_work += now_ns() - arrival;
     }
     done_submitting.store(true);
     completer.join();
@@ -241,7 +219,11 @@ int main(int argc, char** argv) {
            "\"launches\": %lld, \"mean_requests_per_launch\": %.2f, \"books\": %d, "
            "\"catalog_options\": %lld, \"greeks_frac\": %.2f, \"arena_peak_bytes\": %lld, "
            "\"admission_waits\": %lld, \"use_graph\": %d, \"mean_queue_us\": %.2f, "
-           "\"mean_batch_us\": %.2f, \"mean_options_per_launch\": %.0f}\n",
+           "\"mean_batch_us\": %.2f, \"mean_options_per_launch\": %.0f, "
+           "\"dispatch_ms\": {\"admit\": %.1f, \"launch\": %.1f, \"retire\": %.1f, "
+           "\"gpu_idle\": %.1f, \"idle_launches\": %lld}, "
+           "\"client_ms\": {\"reap_wait\": %.1f, \"reap_work\": %.1f, \"submit_wait\": %.1f, "
+           "\"submit_work\": %.1f}}\n",
            o.saturate ? "closed-loop saturation" : "open-loop poisson", o.saturate ? 0.0 : o.rate,
            wall, (long long)requests_done, (long long)options_done, (double)options_done / wall,
            pct(lat_all, 0.50), pct(lat_all, 0.99), pct(lat_small, 0.50), pct(lat_small, 0.99),
@@ -252,6 +234,9 @@ int main(int argc, char** argv) {
            (long long)st.admission_waits, o.use_graph,
            st.requests_done ? (double)st.queue_ns_total * 1e-3 / (double)st.requests_done : 0.0,
            st.launches ? (double)st.batch_ns_total * 1e-3 / (double)st.launches : 0.0,
-           st.launches ? (double)st.options_priced / (double)st.launches : 0.0);
+           st.launches ? (double)st.options_priced / (double)st.launches : 0.0,
+           st.dispatch_admit_ns * 1e-6, st.dispatch_launch_ns * 1e-6, st.dispatch_retire_ns * 1e-6,
+           st.gpu_idle_ns * 1e-6, (long long)st.idle_launches, c_wait * 1e-6, c_work * 1e-6,
+           s_wait * 1e-6, s_work * 1e-6);
     return 0;
 }

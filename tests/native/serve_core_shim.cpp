@@ -3,6 +3,9 @@
 // without a GPU.  Built by tests/native/Makefile; never part of the product.
 #include <stdint.h>
 
+#include <unordered_map>
+#include <vector>
+
 #include "../../p3arsec_amd/csrc/serve_core.hpp"
 
 using bs::serve::Core;
@@ -28,15 +31,16 @@ int sc_submit(void* h, int64_t book, int kind, int64_t now, int64_t* ticket) {
     return static_cast<Core*>(h)->submit(book, kind, now, ticket);
 }
 int sc_cancel(void* h, int64_t t, int64_t now) { return static_cast<Core*>(h)->cancel(t, now); }
-int sc_submit_many(void* h, const int64_t* books, const int* kinds, int64_t count, int64_t now,
-                   int64_t* tickets) {
-    return static_cast<Core*>(h)->submit_many(books, kinds, count, now, tickets);
+int sc_submit_many(void* h, const int64_t* books, const int* kinds, const int64_t* users,
+                   int64_t count, int64_t now, int64_t* tickets) {
+    return static_cast<Core*>(h)->submit_many(books, kinds, users, count, now, tickets);
 }
 int sc_admit_cap(void* h, int64_t now, int64_t cap, int64_t* tickets, int cn) {
-    auto b = static_cast<Core*>(h)->admit(now, cap);
+    std::vector<bs::serve::Admitted> b;
+    static_cast<Core*>(h)->admit(now, &b, cap);
     int i = 0;
-    for (auto* r : b) {
-        if (i < cn) tickets[i] = r->ticket;
+    for (const auto& r : b) {
+        if (i < cn) tickets[i] = r.ticket;
         ++i;
     }
     return i;
@@ -44,7 +48,7 @@ int sc_admit_cap(void* h, int64_t now, int64_t cap, int64_t* tickets, int cn) {
 int64_t sc_queued_options(void* h) { return static_cast<Core*>(h)->queued_options(); }
 void sc_enable_completions(void* h, int on) { static_cast<Core*>(h)->enable_completions(on != 0); }
 // Pops up to max completions into parallel arrays; returns the count.
-int64_t sc_reap(void* h, int64_t* tickets, int* states, int* statuses, int64_t* done,
+int64_t sc_reap(void* h, int64_t* tickets, int* states, int* statuses, int64_t* users,
                 int64_t max) {
     std::vector<bs::serve::Completion> v((size_t)(max > 0 ? max : 0));
     const int64_t k = static_cast<Core*>(h)->reap(v.data(), max);
@@ -52,18 +56,19 @@ int64_t sc_reap(void* h, int64_t* tickets, int* states, int* statuses, int64_t* 
         tickets[i] = v[i].ticket;
         states[i] = v[i].state;
         statuses[i] = v[i].status;
-        done[i] = v[i].t_done;
+        users[i] = v[i].user;
     }
     return k;
 }
 // Admits a batch; writes up to cap tickets and output offsets, returns the count.
 int sc_admit(void* h, int64_t now, int64_t* tickets, int64_t* offs, int cap) {
-    auto b = static_cast<Core*>(h)->admit(now);
+    std::vector<bs::serve::Admitted> b;
+    static_cast<Core*>(h)->admit(now, &b);
     int i = 0;
-    for (auto* r : b) {
+    for (const auto& r : b) {
         if (i < cap) {
-            tickets[i] = r->ticket;
-            offs[i] = (int64_t)r->out_off;
+            tickets[i] = r.ticket;
+            offs[i] = (int64_t)r.out_off;
         }
         ++i;
     }
@@ -100,16 +105,32 @@ void sc_stats(void* h, int64_t* out9) {
 int64_t sc_book_bytes(int64_t n) { return (int64_t)bs::serve::book_bytes(n); }
 int64_t sc_output_bytes(int64_t n, int kind) { return (int64_t)bs::serve::output_bytes(n, kind); }
 
-// Arena alone, for allocator property tests.
-void* sa_create(int64_t cap) { return new bs::serve::Arena((size_t)cap); }
-void sa_destroy(void* a) { delete static_cast<bs::serve::Arena*>(a); }
+// Arena alone, for allocator property tests.  The arena names blocks by
+// handle; the shim maps offsets to handles so tests can free by offset.
+struct ArenaBox {
+    bs::serve::Arena a;
+    std::unordered_map<int64_t, int64_t> live;  // offset -> handle
+    explicit ArenaBox(size_t cap) : a(cap) {}
+};
+void* sa_create(int64_t cap) { return new ArenaBox((size_t)cap); }
+void sa_destroy(void* a) { delete static_cast<ArenaBox*>(a); }
 int sa_alloc(void* a, int64_t bytes, int64_t* off) {
+    auto* x = static_cast<ArenaBox*>(a);
     size_t o = 0;
-    bool ok = static_cast<bs::serve::Arena*>(a)->alloc((size_t)bytes, &o);
+    const int64_t h = x->a.alloc((size_t)bytes, &o);
     *off = (int64_t)o;
+    if (h < 0) return 0;
+    x->live[(int64_t)o] = h;
+    return 1;
+}
+int sa_free(void* a, int64_t off) {
+    auto* x = static_cast<ArenaBox*>(a);
+    auto it = x->live.find(off);
+    if (it == x->live.end()) return 0;
+    const bool ok = x->a.free(it->second);
+    x->live.erase(it);
     return ok ? 1 : 0;
 }
-int sa_free(void* a, int64_t off) { return static_cast<bs::serve::Arena*>(a)->free((size_t)off) ? 1 : 0; }
-int64_t sa_used(void* a) { return (int64_t)static_cast<bs::serve::Arena*>(a)->used(); }
-int64_t sa_largest_free(void* a) { return (int64_t)static_cast<bs::serve::Arena*>(a)->largest_free(); }
+int64_t sa_used(void* a) { return (int64_t)static_cast<ArenaBox*>(a)->a.used(); }
+int64_t sa_largest_free(void* a) { return (int64_t)static_cast<ArenaBox*>(a)->a.largest_free(); }
 }

@@ -217,8 +217,9 @@ def test_submit_many_and_completion_queue(gpu):
             bk = (i64 * len(chunk))(*[ids[i] for i, _ in chunk])
             kd = (ctypes.c_int * len(chunk))(*[k for _, k in chunk])
             tk = (i64 * len(chunk))()
-            gpu.call("bs_submit_many", ctx.h, bk, kd, len(chunk), tk)
-            kinds.update({t: r for t, r in zip(tk, chunk)})
+            us = (i64 * len(chunk))(*range(lo, lo + len(chunk)))
+            gpu.call("bs_submit_many", ctx.h, bk, kd, us, len(chunk), tk)
+            kinds.update({t: (r, u) for t, r, u in zip(tk, chunk, us)})
         cq = (gpu.Completion * 64)()
         cnt = i64()
         seen = []
@@ -229,7 +230,8 @@ def test_submit_many_and_completion_queue(gpu):
             for c in got:
                 assert c.state == gpu.TICKET_DONE and c.status == 0
                 assert c.done_ns >= c.submit_ns > 0
-                i, k = kinds[c.ticket]
+                (i, k), u = kinds[c.ticket]
+                assert c.user == u and c.n == bks[i].n
                 assert same_bits(ctx.result(c.ticket, bks[i].n, k), ref[(i, k)]), (i, k)
             rel = (i64 * max(1, len(got)))(*[c.ticket for c in got])
             gpu.call("bs_result_release_many", ctx.h, rel, len(got))
@@ -241,7 +243,7 @@ def test_submit_many_and_completion_queue(gpu):
         # errors: all-or-nothing submit, double release, reap with max 0
         bk = (i64 * 2)(ids[0], 424242)
         kd = (ctypes.c_int * 2)(1, 1)
-        assert L.bs_submit_many(ctx.h, bk, kd, 2, (i64 * 2)()) == -5
+        assert L.bs_submit_many(ctx.h, bk, kd, None, 2, (i64 * 2)()) == -5
         gpu.call("bs_reap", ctx.h, cq, 64, 0, ctypes.byref(cnt))
         assert cnt.value == 0                         # the failed call queued nothing
         assert L.bs_result_release_many(ctx.h, (i64 * 1)(seen[0]), 1) == -5

@@ -32,7 +32,8 @@ def L():
         "sc_release_book": ([vp, i64], ctypes.c_int),
         "sc_submit": ([vp, i64, ctypes.c_int, i64, P(i64)], ctypes.c_int),
         "sc_cancel": ([vp, i64, i64], ctypes.c_int),
-        "sc_submit_many": ([vp, P(i64), P(ctypes.c_int), i64, i64, P(i64)], ctypes.c_int),
+        "sc_submit_many": ([vp, P(i64), P(ctypes.c_int), P(i64), i64, i64, P(i64)],
+                           ctypes.c_int),
         "sc_enable_completions": ([vp, ctypes.c_int], None),
         "sc_admit_cap": ([vp, i64, i64, P(i64), ctypes.c_int], ctypes.c_int),
         "sc_queued_options": ([vp], i64),
@@ -135,6 +136,13 @@ def test_arena_against_a_model(L):
             else:
                 assert L.sa_largest_free(a) < need
         assert L.sa_used(a) == sum(live.values())
+        if step % 25 == 0:                              # largest free block == largest gap
+            gaps, pos = [], 0
+            for lo in sorted(live):
+                gaps.append(lo - pos)
+                pos = lo + live[lo]
+            gaps.append(cap - pos)
+            assert L.sa_largest_free(a) == max(gaps)
     assert L.sa_free(a, 12345) == 0                     # not a live block
     for off in list(live):
         L.sa_free(a, off)
@@ -247,9 +255,11 @@ def test_never_fitting_request_is_refused(L):
     assert c.submit(b, GREEKS)[0] == BUDGET
 
 
-def reap(c, cap=64):
-    t, s, st, d = (i64 * cap)(), (ctypes.c_int * cap)(), (ctypes.c_int * cap)(), (i64 * cap)()
-    k = c.L.sc_reap(c.h, t, s, st, d, cap)
+def reap(c, cap=64, users=False):
+    t, s, st, u = (i64 * cap)(), (ctypes.c_int * cap)(), (ctypes.c_int * cap)(), (i64 * cap)()
+    k = c.L.sc_reap(c.h, t, s, st, u, cap)
+    if users:
+        return [(t[i], u[i]) for i in range(k)]
     return [(t[i], s[i], st[i]) for i in range(k)]
 
 
@@ -297,16 +307,20 @@ def test_submit_many_is_all_or_nothing(L):
     books = (i64 * 4)(b1, b2, b1, b2)
     kinds = (ctypes.c_int * 4)(PRICE, GREEKS, GREEKS, PRICE)
     tk = (i64 * 4)()
-    assert L.sc_submit_many(c.h, books, kinds, 4, 1, tk) == OK
-    assert list(tk) == sorted(tk) and L.sc_queued(c.h) == 4
+    L.sc_enable_completions(c.h, 1)
+    users = (i64 * 4)(11, 22, 33, 44)
+    assert L.sc_submit_many(c.h, books, kinds, users, 4, 1, tk) == OK
+    assert len(set(tk)) == 4 and L.sc_queued(c.h) == 4
     got, _ = c.admit()
     assert got == list(tk)                             # queued in call order
+    c.complete(got)
+    assert reap(c, users=True) == list(zip(tk, [11, 22, 33, 44]))   # tags come back
     bad_book = (i64 * 3)(b1, 999, b2)
-    assert L.sc_submit_many(c.h, bad_book, kinds, 3, 2, tk) == NOT_FOUND
+    assert L.sc_submit_many(c.h, bad_book, kinds, None, 3, 2, tk) == NOT_FOUND
     bad_kind = (ctypes.c_int * 3)(PRICE, 7, PRICE)
-    assert L.sc_submit_many(c.h, books, bad_kind, 3, 2, tk) == INVALID
+    assert L.sc_submit_many(c.h, books, bad_kind, None, 3, 2, tk) == INVALID
     assert L.sc_queued(c.h) == 0                       # nothing of a failed call queued
-    assert L.sc_submit_many(c.h, books, kinds, 0, 2, tk) == OK
+    assert L.sc_submit_many(c.h, books, kinds, None, 0, 2, tk) == OK
 
 
 def test_admit_option_cap_and_queued_options(L):
@@ -331,3 +345,41 @@ def test_admit_option_cap_and_queued_options(L):
     c.submit(b)
     L.sc_fail_queued(c.h, SHUTDOWN, 9)
     assert L.sc_queued_options(c.h) == 0
+
+
+def test_arena_size_classes_and_reuse(L):
+    """Sizes across the allocator's classes (256 B to 64 MiB), including
+    sizes just above a class edge, which the O(1) search skips: a hole
+    between live blocks is found when it is the only block that fits (the
+    exact search of the request's own class); neighbours coalesce in either
+    order; an exactly filled arena refuses even one byte."""
+    cap = 1 << 30
+    a = L.sa_create(cap)
+    o = [i64() for _ in range(5)]
+    sizes = [256, 4096, 15 * 256, 16 * 256, 17 * 256, 1 << 20, (1 << 20) + 256,
+             (1 << 20) + 255 * 256, 64 << 20]
+    for sz in sizes:
+        assert L.sa_alloc(a, sz, ctypes.byref(o[0])) == 1
+        assert L.sa_alloc(a, sz, ctypes.byref(o[1])) == 1
+        assert L.sa_alloc(a, sz, ctypes.byref(o[2])) == 1
+        rest = L.sa_largest_free(a)
+        assert rest == cap - 3 * ((sz + 255) // 256 * 256)
+        assert L.sa_alloc(a, rest, ctypes.byref(o[4])) == 1    # fill the arena
+        assert L.sa_largest_free(a) == 0
+        assert L.sa_free(a, o[1].value) == 1                   # hole between live blocks
+        assert L.sa_alloc(a, sz + 256, ctypes.byref(o[3])) == 0  # one unit too big
+        assert L.sa_alloc(a, sz, ctypes.byref(o[3])) == 1
+        assert o[3].value == o[1].value                        # the only fit is used
+        for x in (o[0], o[4], o[2], o[3]):
+            assert L.sa_free(a, x.value) == 1
+        assert L.sa_used(a) == 0 and L.sa_largest_free(a) == cap
+    # fill exactly, then free in interleaved order
+    offs = []
+    for _ in range(16):
+        assert L.sa_alloc(a, cap // 16, ctypes.byref(o[0])) == 1
+        offs.append(o[0].value)
+    assert L.sa_largest_free(a) == 0 and L.sa_alloc(a, 1, ctypes.byref(o[1])) == 0
+    for off in offs[::2] + offs[1::2]:
+        assert L.sa_free(a, off) == 1
+    assert L.sa_largest_free(a) == cap
+    L.sa_destroy(a)
