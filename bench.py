@@ -7,7 +7,10 @@ A step is one pass of the price map over the 10,000,000-option FP32 book
 option), one kernel launch per pass, as the reference's NUM_RUNS loop
 (blackscholes.c:316-347).  Inputs are resident in HBM before timing starts.
 Passes rotate over --books device copies of the book (4 x 250 MB), so the
-256 MB MALL cannot hold the working set between passes.
+256 MB MALL cannot hold the working set between passes, and alternate
+between two HIP streams, so the last waves of one pass overlap the first
+waves of the next (passes are independent: adjacent passes use different
+books); the same passes on one stream are reported beside it.
 
 Prints one JSON line: options/s over the timed passes, the roofline of the
 price kernel (algorithmic bytes / average launch time vs 8 TB/s), and the
@@ -55,23 +58,63 @@ def measured_traffic(kernel: str, n: int):
     return rec["traffic_bytes"]
 
 
-def timed_passes(launch, nbooks: int, steps: int, warmup: int, stream) -> float:
-    """Runs warmup + steps launches (rotating books); returns ms per step of the
-    timed steps, from HIP events on the launch stream."""
+LAUNCH_CPU_US = {}
+# Streams the passes of each map alternate over, the faster schedule measured
+# on MI355X (10M options): price 37.9 us per pass on two streams vs 40.5 on
+# one; FP64 109.5 vs 113; the write-heavy Greeks map 70.5 us on one stream vs
+# 73.3 on two.
+STREAMS = {"price": 2, "fast": 2, "greeks": 1, "fp64": 2}
+
+
+def spin_up(seconds: float = 0.25) -> None:
+    """Brings the GPU to its working clocks before anything is timed: plain
+    256 MiB device-to-device copies for `seconds` (the clocks need about 30 ms
+    of load to settle; a timed region that starts cold reads 10-20% slow, and
+    so does a profiler's per-kernel average over a cold warm-up)."""
+    nbytes = 256 << 20
+    src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+    t0 = time.time()
+    while time.time() - t0 < seconds:
+        for _ in range(20):
+            capi.call("bs_memcpy_d2d", dst.ptr, src.ptr, nbytes, None)
+        capi.call("bs_device_sync")
+    src.free()
+    dst.free()
+
+
+def timed_passes(launch, nbooks: int, steps: int, warmup: int, streams, tag: str = "") -> float:
+    """Runs warmup + steps passes, pass i on book i % nbooks and stream
+    i % len(streams) (launch(i, stream)); returns ms per pass of the timed
+    passes.  The timed region is bracketed by HIP events on the first stream:
+    the other streams wait for the start event, and the end event waits for
+    every stream's last pass.  Passes on different streams touch different
+    books and outputs (nbooks is a multiple of the stream count), so with two
+    streams one pass's last waves overlap the next pass's first ones.  Also
+    records the host time per launch call (LAUNCH_CPU_US[tag])."""
+    assert nbooks % len(streams) == 0
     for i in range(warmup):
-        launch(i % nbooks)
-    capi.call("bs_stream_sync", stream)
-    e0, e1 = capi.Event(), capi.Event()
+        launch(i % nbooks, streams[i % len(streams)].handle)
     capi.call("bs_device_sync")
-    e0.record(stream)
+    e0, e1 = capi.Event(), capi.Event()
+    joins = [capi.Event() for _ in streams[1:]]
+    e0.record(streams[0].handle)
+    for s in streams[1:]:
+        s.wait(e0)
+    t0 = time.perf_counter()
     for i in range(steps):
-        launch(i % nbooks)
-    e1.record(stream)
+        launch(i % nbooks, streams[i % len(streams)].handle)
+    t1 = time.perf_counter()
+    for s, ev in zip(streams[1:], joins):
+        ev.record(s.handle)
+        streams[0].wait(ev)
+    e1.record(streams[0].handle)
     e1.sync()
     capi.call("bs_device_sync")
     ms = e0.elapsed_ms(e1) / steps
-    e0.destroy()
-    e1.destroy()
+    if tag:
+        LAUNCH_CPU_US[tag] = (t1 - t0) / steps * 1e6
+    for ev in [e0, e1] + joins:
+        ev.destroy()
     return ms
 
 
@@ -119,7 +162,7 @@ def serve_bench(result: dict) -> dict:
     return out
 
 
-def only_one(args, L, host, dbooks, outs, stream) -> int:
+def only_one(args, L, host, dbooks, outs, streams) -> int:
     """Runs one kernel for --warmup + --steps passes (profiler runs)."""
     n, nb = host.n, len(dbooks)
     extra = []
@@ -129,29 +172,30 @@ def only_one(args, L, host, dbooks, outs, stream) -> int:
         nbytes = 256 << 20
         src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
         capi.call("bs_memset", src.ptr, 1, nbytes, None)
-        fn = lambda i: L.bs_memcpy_d2d(dst.ptr, src.ptr, nbytes, stream.handle)  # noqa
-        ms = timed_passes(lambda i: capi.check(fn(i), "copy"), 1, args.steps, args.warmup,
-                          stream.handle)
+        fn = lambda i, s: L.bs_memcpy_d2d(dst.ptr, src.ptr, nbytes, s)  # noqa
+        ms = timed_passes(lambda i, s: capi.check(fn(i, s), "copy"), 1, args.steps, args.warmup,
+                          streams[:1])
         print(json.dumps({"kernel": "copy", "bytes": nbytes, "ms_per_step": ms,
                           "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return 0
     if args.only == "price":
-        fn = lambda i: L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle)  # noqa
+        fn = lambda i, s: L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s)  # noqa
     elif args.only == "fast":
-        fn = lambda i: L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle)  # noqa
+        fn = lambda i, s: L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s)  # noqa
     elif args.only == "greeks":
         extra = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
-        fn = lambda i: L.bs_price_greeks_f32(*dbooks[i].ptrs(),  # noqa
-                                             *[o.ptr for o in extra[i]], n, stream.handle)
+        fn = lambda i, s: L.bs_price_greeks_f32(*dbooks[i].ptrs(),  # noqa
+                                                *[o.ptr for o in extra[i]], n, s)
     else:
         h64 = host.astype(np.float64)
         d64 = [capi.DeviceBook(h64, dtype=np.float64) for _ in range(2)]
         o64 = [capi.alloc_out(n, np.float64) for _ in range(2)]
         extra = [d64, o64]
         nb = 2
-        fn = lambda i: L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, stream.handle)  # noqa
-    ms = timed_passes(lambda i: capi.check(fn(i), args.only), nb, args.steps, args.warmup,
-                      stream.handle)
+        fn = lambda i, s: L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, s)  # noqa
+    ss = streams[:1] if args.one_stream else streams[:STREAMS[args.only]]
+    ms = timed_passes(lambda i, s: capi.check(fn(i, s), args.only), nb, args.steps, args.warmup,
+                      ss)
     print(json.dumps({"kernel": args.only, "options": n, "ms_per_step": ms}), flush=True)
     return 0
 
@@ -160,7 +204,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--books", type=int, default=4)
     ap.add_argument("--cpu-passes", type=int, default=20)
@@ -170,6 +214,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy"],
                     help="time one kernel only (for profiler runs)")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="with --only: every pass on one stream")
     args = ap.parse_args()
     global ARGS
     ARGS = args
@@ -189,16 +235,20 @@ def main() -> int:
     host = books.make_book(n, books.SEED_BASE + 2)
     dbooks = [capi.DeviceBook(host) for _ in range(nb)]
     outs = [capi.alloc_out(n) for _ in range(nb)]
-    stream = capi.Stream()
+    streams = [capi.Stream(), capi.Stream()]
     L = capi.load()
 
-    def launch_price(i):
-        capi.check(L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle), "price")
+    def launch_price(i, s):
+        capi.check(L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s), "price")
 
+    spin_up()
     if args.only:
-        return only_one(args, L, host, dbooks, outs, stream)
+        return only_one(args, L, host, dbooks, outs, streams)
     log("bench: price map")
-    ms = timed_passes(launch_price, nb, args.steps, args.warmup, stream.handle)
+    ms = timed_passes(launch_price, nb, args.steps, args.warmup, streams[:STREAMS["price"]],
+                      "price")
+    # the same passes on one stream (each pass waits for the previous one)
+    ms_one = timed_passes(launch_price, nb, args.steps, min(args.warmup, 200), streams[:1])
     value = n / (ms * 1e-3)
     alg_bytes = 25 * n
     achieved = alg_bytes / (ms * 1e-3) / 1e9
@@ -221,28 +271,32 @@ def main() -> int:
         "config": {"workload": "Black-Scholes price map, 10M-option FP32 book, SoA "
                                "(5 x f32 + u8 in, f32 out), one launch per pass",
                    "options": n, "rotating_books": nb, "bytes_per_option": 25,
-                   "max_abs_err_vs_oracle_sample": max_err},
+                   "streams": STREAMS["price"], "max_abs_err_vs_oracle_sample": max_err},
+        "one_stream": {"ms_per_step": ms_one, "options_per_s": n / (ms_one * 1e-3),
+                       "roofline_frac": alg_bytes / (ms_one * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("price", n)},
     }
 
     if not args.no_extra:
-        def launch_fast(i):
-            capi.check(L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, stream.handle),
-                       "fast")
+        def launch_fast(i, s):
+            capi.check(L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s), "fast")
         log("bench: fast price map")
-        fast_ms = timed_passes(launch_fast, nb, args.steps, args.warmup, stream.handle)
+        fast_ms = timed_passes(launch_fast, nb, args.steps, min(args.warmup, 200),
+                               streams[:STREAMS["fast"]])
         result["fast_variant"] = {"ms_per_step": fast_ms, "options_per_s": n / (fast_ms * 1e-3),
                                   "achieved_gbs": alg_bytes / (fast_ms * 1e-3) / 1e9}
         gouts = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
 
-        def launch_greeks(i):
+        def launch_greeks(i, s):
             capi.check(L.bs_price_greeks_f32(*dbooks[i].ptrs(), *[o.ptr for o in gouts[i]],
-                                             n, stream.handle), "greeks")
+                                             n, s), "greeks")
         log("bench: greeks map")
-        gms = timed_passes(launch_greeks, nb, max(10, args.steps // 2), 5, stream.handle)
+        gms = timed_passes(launch_greeks, nb, max(10, args.steps // 2), min(args.warmup, 200),
+                           streams[:STREAMS["greeks"]], "greeks")
         result["greeks"] = {"ms_per_step": gms, "ratio_to_price": gms / ms,
+                            "ratio_to_price_one_stream": gms / ms_one,
                             "options_per_s": n / (gms * 1e-3),
                             "achieved_gbs": 41 * n / (gms * 1e-3) / 1e9}
         for row in gouts:
@@ -254,9 +308,10 @@ def main() -> int:
         d64 = [capi.DeviceBook(h64, dtype=np.float64) for _ in range(2)]
         o64 = [capi.alloc_out(n, np.float64) for _ in range(2)]
 
-        def launch_f64(i):
-            capi.check(L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, stream.handle), "f64")
-        fms = timed_passes(launch_f64, 2, max(10, args.steps // 4), 3, stream.handle)
+        def launch_f64(i, s):
+            capi.check(L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, s), "f64")
+        fms = timed_passes(launch_f64, 2, max(10, args.steps // 4), min(args.warmup, 100),
+                           streams[:STREAMS["fp64"]])
         result["fp64"] = {"ms_per_step": fms, "ratio_to_fp32": fms / ms,
                           "options_per_s": n / (fms * 1e-3),
                           "achieved_gbs": 49 * n / (fms * 1e-3) / 1e9}
@@ -273,12 +328,14 @@ def main() -> int:
         log("bench: cpu baseline")
         result["cpu_baseline"] = cpu_baseline(host, args.cpu_passes)
 
+    result["launch_cpu_us_per_step"] = LAUNCH_CPU_US
     print(json.dumps(result), flush=True)
     for b in dbooks:
         b.free()
     for o in outs:
         o.free()
-    stream.destroy()
+    for s in streams:
+        s.destroy()
     return 0
 
 

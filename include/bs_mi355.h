@@ -61,7 +61,7 @@ int bs_version(void); /* 10000*major + 100*minor + patch */
 enum {
     BS_CFG_CHECK_POINTERS = 1, /* 1 (default): validate device pointers      */
     BS_CFG_MAX_BLOCKS = 2,     /* grid cap for streaming kernels, 0 = auto    */
-    BS_CFG_BLOCK_THREADS = 3,  /* read-only: threads per block (256)          */
+    BS_CFG_BLOCK_THREADS = 3,  /* streaming block size: 0 (default), 64..256 */
     BS_CFG_VECTOR_WIDTH = 4,   /* options per work-item: 0 (default), 2, 4   */
     BS_CFG_NONTEMPORAL = 5,    /* 1 (default): non-temporal loads and stores  */
     BS_CFG_PIPELINE = 6        /* -1 (default), 0, 1: software pipelining     */
@@ -161,6 +161,8 @@ int bs_event_create(bs_event_t *ev);
 int bs_event_destroy(bs_event_t ev);
 int bs_event_record(bs_event_t ev, bs_stream_t stream);
 int bs_event_sync(bs_event_t ev);
+/* Work queued on `stream` after this call waits for `ev`'s last record. */
+int bs_stream_wait_event(bs_stream_t stream, bs_event_t ev);
 int bs_event_elapsed_ms(bs_event_t start, bs_event_t stop, float *ms);
 
 /* ---- serving: resident books, continuous batching -------------------------

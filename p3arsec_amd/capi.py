@@ -97,6 +97,7 @@ _SIGS = {
     "bs_stream_create": ([ctypes.POINTER(_vp)], _i32),
     "bs_stream_destroy": ([_vp], _i32),
     "bs_stream_sync": ([_vp], _i32),
+    "bs_stream_wait_event": ([_vp, _vp], _i32),
     "bs_device_sync": ([], _i32),
     "bs_event_create": ([ctypes.POINTER(_vp)], _i32),
     "bs_event_destroy": ([_vp], _i32),
@@ -232,6 +233,9 @@ class Stream:
 
     def sync(self) -> None:
         call("bs_stream_sync", self.handle)
+
+    def wait(self, ev: "Event") -> None:
+        call("bs_stream_wait_event", self.handle, ev.handle)
 
     def destroy(self) -> None:
         if self.handle:

@@ -144,7 +144,12 @@ int bs_config_set(int key, int64_t value) {
         case BS_CFG_PIPELINE:
             config().pipeline = value < 0 ? -1 : (value != 0);
             return BS_OK;
-        default: return set_error(BS_ERR_INVALID, "unknown or read-only config key");
+        case BS_CFG_BLOCK_THREADS:
+            if (value != 0 && value != 64 && value != 128 && value != 256)
+                return set_error(BS_ERR_INVALID, "block threads must be 0, 64, 128 or 256");
+            config().block_threads = (int)value;
+            return BS_OK;
+        default: return set_error(BS_ERR_INVALID, "unknown config key");
     }
 }
 
@@ -153,7 +158,7 @@ int bs_config_get(int key, int64_t* value) {
     switch (key) {
         case BS_CFG_CHECK_POINTERS: *value = config().check_pointers; return BS_OK;
         case BS_CFG_MAX_BLOCKS: *value = config().max_blocks; return BS_OK;
-        case BS_CFG_BLOCK_THREADS: *value = kThreads; return BS_OK;
+        case BS_CFG_BLOCK_THREADS: *value = config().block_threads; return BS_OK;
         case BS_CFG_VECTOR_WIDTH: *value = config().vector_width; return BS_OK;
         case BS_CFG_NONTEMPORAL: *value = config().nontemporal; return BS_OK;
         case BS_CFG_PIPELINE: *value = config().pipeline; return BS_OK;
@@ -417,6 +422,11 @@ int bs_stream_sync(bs_stream_t stream) {
     return done(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
 }
 int bs_device_sync(void) { return done(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+int bs_stream_wait_event(bs_stream_t stream, bs_event_t ev) {
+    if (!ev) return set_error(BS_ERR_INVALID, "ev is NULL");
+    return done(hipStreamWaitEvent(as_stream(stream), static_cast<hipEvent_t>(ev), 0),
+                "hipStreamWaitEvent");
+}
 int bs_event_create(bs_event_t* ev) {
     if (!ev) return set_error(BS_ERR_INVALID, "ev is NULL");
     hipEvent_t e = nullptr;

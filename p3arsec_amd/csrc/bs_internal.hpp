@@ -17,6 +17,7 @@ struct Config {
     int vector_width = 0;  // 0: each kernel's measured default (2 or 4)
     int nontemporal = 1;
     int pipeline = -1;     // -1: each kernel's measured default
+    int block_threads = 0;  // streaming kernels' block size: 0 = each kernel's default
 };
 Config& config();
 

@@ -72,8 +72,8 @@ Op make_op(const CndfArgs<double>& a) { return Op{a.x, a.out}; }
 template <class Op, bool NT, bool PIPE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, PIPE>())))
 void k_stream(Op op, int64_t ngroups, int64_t tail_begin, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * kThreads;
-    const int64_t gid = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t g = gid;
     if constexpr (PIPE) {
         if (g < ngroups) {
@@ -107,8 +107,8 @@ inline bool aligned(const void* p, uintptr_t a) {
     return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
 }
 
-inline unsigned grid_for(int64_t work_items, int64_t cap) {
-    int64_t blocks = (work_items + kThreads - 1) / kThreads;
+inline unsigned grid_for(int64_t work_items, int64_t cap, int threads = kThreads) {
+    int64_t blocks = (work_items + threads - 1) / threads;
     if (cap > 0 && blocks > cap) blocks = cap;
     if (blocks > 0x7fffffff) blocks = 0x7fffffff;
     return (unsigned)(blocks < 1 ? 1 : blocks);
@@ -119,8 +119,9 @@ hipError_t launch_form(const Op& op, int64_t n, hipStream_t st) {
     constexpr int VW = Op::kVW;
     const int64_t ngroups = n / VW;
     const int64_t cap = config().max_blocks > 0 ? config().max_blocks : Op::kBlocks;
-    const unsigned grid = grid_for(std::max<int64_t>(ngroups, 1), cap);
-    hipLaunchKernelGGL((k_stream<Op, NT, PIPE>), dim3(grid), dim3(kThreads), 0, st, op, ngroups,
+    const int nt = config().block_threads > 0 ? config().block_threads : Op::kThreadsPref;
+    const unsigned grid = grid_for(std::max<int64_t>(ngroups, 1), cap, nt);
+    hipLaunchKernelGGL((k_stream<Op, NT, PIPE>), dim3(grid), dim3(nt), 0, st, op, ngroups,
                        ngroups * VW, n);
     return hipGetLastError();
 }

@@ -12,14 +12,18 @@
 // reproducing its rounding sequence.  Three arithmetic policies:
 //
 //   Ref32    (default)  float, the reference's statement order and roundings:
-//            correctly rounded division and square root (hardware seed plus
-//            one residual correction), products with the double literals
-//            rounded once (a*c_hi + a*c_lo in one FMA), exp from v_exp_f32
-//            with a two-float argument (within one ulp of glibc's expf) and
-//            log from v_log_f32 (within two ulps).  Measured and emulated over
-//            10M-option books, the transcendental ulps keep every price within
-//            9e-5 of the reference, while correctly rounded division, square
-//            root and literal products are what keep it under 1e-4.
+//            correctly rounded division, reciprocal and square root (hardware
+//            seed plus one residual correction), products with the double
+//            literals rounded once (a*c_hi + a*c_lo in one FMA), exp and log
+//            from v_exp_f32 and v_log_f32.  Each refinement was switched off
+//            in turn over five 10M-option books plus the edge and golden books
+//            (50M options): with all of them the largest difference from the
+//            reference is 8.39e-5 and none is above 9e-5; dropping the
+//            correctly rounded reciprocal, the literal products, or division
+//            and square root together takes it over 1e-4; dropping square
+//            root, division or the reference's polynomial order alone takes
+//            it to 9.5e-5..9.9e-5; a compensated exp (Cody-Waite argument)
+//            changes nothing, so exp is the plain hardware one.
 //   Fast32   float, hardware transcendentals (v_exp_f32, v_log_f32,
 //            v_rcp_f32, v_sqrt_f32), Horner polynomial with FMA.  About a
 //            third of Ref32's VALU work; prices differ from the reference by
@@ -98,21 +102,11 @@ __device__ __forceinline__ float mul_dc(float a, SplitConst c) {
     return __builtin_fmaf(a, c.hi, a * c.lo);
 }
 
-// exp(x) = 2^th e^r with th = round(x log2(e)) and r = x - th ln2 carried
-// exactly (Cody-Waite: ln2 split in two floats, each product in an FMA), then
-// e^r = 1 + r to float precision since |r| <= 2^-23 |x|.  Measured on gfx950:
-// 87% of results equal the correctly rounded float, the rest are one ulp
-// away.  The argument is clamped to [-150, 89] (results 0 and +inf outside,
-// as expf gives).
-constexpr float kLn2Hi = 0.693145751953125f;        // 11 significant bits
-constexpr float kLn2Lo = 1.428606820309417232e-06f;  // ln2 - kLn2Hi
-__device__ __forceinline__ float exp_hw(float x) {
-    x = __builtin_amdgcn_fmed3f(x, -150.0f, 89.0f);
-    const float th = x * kLog2e;
-    const float r = __builtin_fmaf(-th, kLn2Lo, __builtin_fmaf(-th, kLn2Hi, x));
-    const float e = __builtin_amdgcn_exp2f(th);
-    return __builtin_fmaf(e, r, e);
-}
+// exp(x) = 2^(x log2 e) on v_exp_f32.  The rounding of x log2 e costs up to
+// |x| 2^-24 relative, which is large only where the result is tiny: the
+// price path's exponents are -d^2/2 (the result scales a CNDF tail that is
+// itself below 1e-5 once |x| > 12) and -rT (|x| <= 0.2).
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * kLog2e); }
 
 // log(x) = log2(x) ln2 from the hardware v_log_f32 (70% correctly rounded,
 // the rest within two ulps on the S/K quotients of a book).

@@ -8,6 +8,12 @@
 // per SIMD = 64 VGPRs), kBlocks (default grid cap, 0 = one group per
 // work-item) and kPipe (default loop form); the defaults and each launcher's
 // preferred width are the fastest forms measured on MI355X for 10M options.
+// Defaults measured on MI355X, 10M options, warm clocks (block size x loop
+// form): price 64 threads pipelined 39.3 us (256 threads: 43.0), int32-type
+// price 64 threads 44.9 us, FP64 64 threads pipelined 108 us, Greeks 256
+// threads pipelined 70.7 us (64 threads: 74.2).  Small blocks help the
+// price streams: one-wave workgroups retire and refill independently, so the
+// tail of the grid is short.
 #pragma once
 #include <type_traits>
 
@@ -42,6 +48,7 @@ template <class Math, int VW>
 struct PriceF32 {
     static constexpr int kVW = VW;
     static constexpr int kWaves = 8;
+    static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
     static constexpr bool kPipe = true;
     using F = vec<float, VW>;
@@ -80,6 +87,7 @@ template <int VW>
 struct PriceI32F32 {
     static constexpr int kVW = VW;
     static constexpr int kWaves = 7;
+    static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
     static constexpr bool kPipe = true;
     using F = vec<float, VW>;
@@ -120,6 +128,7 @@ template <int VW>
 struct PriceF64 {
     static constexpr int kVW = 2;
     static constexpr int kWaves = 5;
+    static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
     static constexpr bool kPipe = true;
     using D = vec<double, 2>;
@@ -158,8 +167,9 @@ template <int VW>
 struct GreeksF32 {
     static constexpr int kVW = VW;
     static constexpr int kWaves = VW == 4 ? 5 : 8;
+    static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
-    static constexpr bool kPipe = false;
+    static constexpr bool kPipe = true;
     using F = vec<float, VW>;
     SoaIn<float> in;
     const uint8_t* type;
@@ -213,6 +223,7 @@ template <int VW>
 struct CndfF32 {
     static constexpr int kVW = VW;
     static constexpr int kWaves = 8;
+    static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
     static constexpr bool kPipe = false;
     using F = vec<float, VW>;
@@ -241,6 +252,7 @@ template <int VW>
 struct CndfF64 {
     static constexpr int kVW = 2;
     static constexpr int kWaves = 8;
+    static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
     static constexpr bool kPipe = false;
     using D = vec<double, 2>;

@@ -41,9 +41,13 @@ def test_config_roundtrip_and_validation():
     assert capi.get_config(capi.CFG_VECTOR_WIDTH) == 2
     capi.set_config(capi.CFG_VECTOR_WIDTH, old)
     assert L.bs_config_set(capi.CFG_VECTOR_WIDTH, 3) == -1
-    assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 64) == -1   # read-only
+    assert capi.get_config(capi.CFG_BLOCK_THREADS) == 0        # each kernel's default
+    capi.set_config(capi.CFG_BLOCK_THREADS, 64)
+    assert capi.get_config(capi.CFG_BLOCK_THREADS) == 64
+    assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 100) == -1  # 0, 64, 128 or 256
+    assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 512) == -1
+    capi.set_config(capi.CFG_BLOCK_THREADS, 0)
     assert L.bs_config_set(999, 1) == -1
-    assert capi.get_config(capi.CFG_BLOCK_THREADS) == 256
 
 
 def test_argument_errors_before_device_work():

@@ -37,6 +37,7 @@ def reset_config(capi):
     capi.set_config(capi.CFG_VECTOR_WIDTH, 0)
     capi.set_config(capi.CFG_NONTEMPORAL, 1)
     capi.set_config(capi.CFG_PIPELINE, -1)
+    capi.set_config(capi.CFG_BLOCK_THREADS, 0)
 
 
 def err_stats(got, ref):
@@ -90,7 +91,8 @@ def test_launch_shape_and_alignment_do_not_change_bits(gpu):
     try:
         for key, val in ((gpu.CFG_MAX_BLOCKS, 7), (gpu.CFG_VECTOR_WIDTH, 2),
                          (gpu.CFG_VECTOR_WIDTH, 4), (gpu.CFG_NONTEMPORAL, 0),
-                         (gpu.CFG_PIPELINE, 1), (gpu.CFG_PIPELINE, 0)):
+                         (gpu.CFG_PIPELINE, 1), (gpu.CFG_PIPELINE, 0),
+                         (gpu.CFG_BLOCK_THREADS, 64), (gpu.CFG_BLOCK_THREADS, 128)):
             gpu.set_config(key, val)
             variants.append(run(gpu, book))
             reset_config(gpu)
