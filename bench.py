@@ -319,6 +319,21 @@ def main() -> int:
             b.free()
         for o in o64:
             o.free()
+        # host-buffer boundary (the SkePU2 Map form): upload, one pass, download
+        log("bench: host-buffer map")
+        arrs = [np.ascontiguousarray(a) for a in host.arrays()[:5]]
+        ty32 = host.otype.astype(np.int32)
+        hout = np.empty(n, np.float32)
+        hts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            capi.check(L.bs_price_host_f32(*[a.ctypes.data for a in arrs], ty32.ctypes.data,
+                                           hout.ctypes.data, n, 1, 0), "host")
+            hts.append(time.perf_counter() - t0)
+        result["host_buffers"] = {"seconds_per_call": min(hts),
+                                  "options_per_s": n / min(hts),
+                                  "note": "pageable host arrays in and out (PCIe-inclusive), "
+                                          "one pass; not the headline value"}
 
     if not args.no_extra and not args.no_serve:
         log("bench: serving loop")
