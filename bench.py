@@ -42,6 +42,16 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def profile_record(kernel: str, n: int) -> dict:
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return {}
+    return rec if rec.get("options") == n else {}
+
+
 def measured_traffic(kernel: str, n: int):
     """HBM bytes per launch of `kernel` on the n-option book, from the committed
     rocprofv3 counter runs (profiles/hbm_traffic.json: FETCH_SIZE x 2 +
@@ -125,13 +135,24 @@ def cpu_baseline(book_host, passes: int) -> dict:
         aff = len(os.sched_getaffinity(0))
     except AttributeError:
         aff = os.cpu_count() or 1
-    threads = max(1, min(aff, 16))
+    threads = max(1, min(aff, 16))     # the box's CPU share for one GPU
     _, sec, used = pyoracle.price_f32(book_host, nthreads=threads, passes=passes)
     rate = book_host.n * passes / sec
+    one = book_host.slice(0, 1_000_000)
+    _, sec1, _ = pyoracle.price_f32(one, nthreads=1, passes=2)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": rate, "unit": "options/s", "cores": used, "kind": "port",
             "sample": f"{book_host.n} options x {passes} passes, static blocks, "
                       f"{used} pinned threads, oracle/bs_oracle.c (bit-exact with "
-                      f"blackscholes.c), {sec:.2f} s"}
+                      f"blackscholes.c), {sec:.2f} s",
+            "single_thread_options_per_s": one.n * 2 / sec1,
+            "speedup": rate / (one.n * 2 / sec1),
+            "cpu_model": model, "nproc": os.cpu_count(), "affinity": aff}
 
 
 def serve_bench(result: dict) -> dict:
@@ -278,6 +299,13 @@ def main() -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("price", n)},
     }
+    prof = profile_record("price", n)
+    if "valu_issue_us" in prof:
+        # VALU busy time per pass from the committed SQ_INSTS_VALU count (wave64
+        # instructions x 4 cycles over 1024 SIMDs at 2.4 GHz), against this pass time
+        result["valu"] = {"issue_us_per_pass": prof["valu_issue_us"],
+                          "frac_of_pass": prof["valu_issue_us"] / (ms * 1e3),
+                          "source": "profiles/hbm_traffic.json (rocprofv3 --pmc SQ_INSTS_VALU)"}
 
     if not args.no_extra:
         def launch_fast(i, s):
@@ -319,6 +347,53 @@ def main() -> int:
             b.free()
         for o in o64:
             o.free()
+        # portfolio sums over the same book (the MapReduce step after the map):
+        # 21 B/option read, nothing per option written
+        log("bench: portfolio sums")
+        sums = [capi.DeviceBuffer(40) for _ in range(nb)]
+
+        def launch_pf(i, s, greeks):
+            capi.check(L.bs_portfolio_f32(*dbooks[i].ptrs(), None, n, greeks, sums[i].ptr, s),
+                       "portfolio")
+        tpf = timed_passes(lambda i, s: launch_pf(i, s, 0), nb, max(10, args.steps // 2),
+                           min(args.warmup, 200), streams[:STREAMS["price"]])
+        tpg = timed_passes(lambda i, s: launch_pf(i, s, 1), nb, max(10, args.steps // 4),
+                           min(args.warmup, 100), streams[:STREAMS["price"]])
+        result["portfolio"] = {"price_sum_us": tpf * 1e3, "price_sum_gbs": 21 * n / (tpf * 1e-3) / 1e9,
+                               "greeks_sums_us": tpg * 1e3,
+                               "greeks_sums_gbs": 21 * n / (tpg * 1e-3) / 1e9}
+        for b in sums:
+            b.free()
+        # the 4096-option book of the reference's CPU-runnable case: L2-resident, so a
+        # per-pass latency, not an HBM figure; error against the FP64 closed form
+        log("bench: 4096-option book")
+        small = books.make_book(4096, books.SEED_BASE + 1)
+        dsm, osm = capi.DeviceBook(small), capi.alloc_out(4096)
+        t4 = timed_passes(lambda i, s: capi.check(L.bs_price_f32(*dsm.ptrs(), osm.ptr, 4096, s),
+                                                  "4096"), 1, 100, 100, streams[:1])
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        p4 = osm.download(np.float32, 4096).astype(np.float64)
+        e4 = np.abs(p4 - pyoracle.price_exact(small))
+        r4, _, _ = pyoracle.price_f32(small)
+        result["config_4096"] = {"us_per_pass": t4 * 1e3, "options_per_s": 4096 / (t4 * 1e-3),
+                                 "passes": 100, "max_err_vs_fp64_closed_form": float(e4.max()),
+                                 "mean_err_vs_fp64_closed_form": float(e4.mean()),
+                                 "max_abs_diff_vs_reference": float(np.abs(p4 - r4).max())}
+        dsm.free()
+        osm.free()
+        # one GPU's share of an 8-way sharded book (10M/8 options, one book, not
+        # rotated): the working set fits the 256 MB MALL, so this is not HBM-bound
+        log("bench: 1.25M-option shard")
+        ns = n // 8
+        shard = capi.DeviceBook(host.slice(0, ns))
+        oshard = capi.alloc_out(ns)
+        tsh = timed_passes(lambda i, s: capi.check(L.bs_price_f32(*shard.ptrs(), oshard.ptr, ns, s),
+                                                   "shard"), 1, 200, 200, streams[:1])
+        result["shard_1_25m"] = {"us_per_pass": tsh * 1e3, "options_per_s": ns / (tsh * 1e-3),
+                                 "note": "one book reused every pass: MALL-resident working set"}
+        shard.free()
+        oshard.free()
         # host-buffer boundary (the SkePU2 Map form): upload, one pass, download
         log("bench: host-buffer map")
         arrs = [np.ascontiguousarray(a) for a in host.arrays()[:5]]

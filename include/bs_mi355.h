@@ -120,6 +120,17 @@ int bs_price_greeks_f32(const float *s, const float *k, const float *r, const fl
                         float *vega, float *rho, float *theta, int64_t n,
                         bs_stream_t stream);
 
+/* Portfolio reduction over a book: sums[0] = sum of w_i * price_i and, with
+ * greeks != 0, sums[1..4] = the sums of w_i * delta_i, vega_i, rho_i, theta_i,
+ * each per-option value exactly the one bs_price_f32 / bs_price_greeks_f32
+ * computes.  weights may be NULL (every w_i = 1).  sums is a device array of 1
+ * or 5 doubles.  Summed in FP64 in a fixed order: bit-reproducible for a given
+ * n.  n == 0 writes zeros.  No reference counterpart: the MapReduce step that
+ * follows the map (SkePU2 MapReduce, skepu2/backend/impl/mapreduce/). */
+int bs_portfolio_f32(const float *s, const float *k, const float *r, const float *v,
+                     const float *t, const uint8_t *otype, const float *weights, int64_t n,
+                     int greeks, double *sums, bs_stream_t stream);
+
 /* AoS -> SoA repack on the device (BS:747-767).  `aos` points at n records
  * laid out as the reference's OptionData (BS:89-100): float s, strike, r,
  * divq, v, t; char OptionType; float divs, DGrefval -- 36 bytes each.  The
