@@ -6,14 +6,15 @@
 // the map computes (the same per-group code, bs_ops.hpp) and w_i is the
 // position (1 when no weights are given).
 //
-// Two kernels on the caller's stream.  k_pf_blocks: each work-item walks its
-// 2-option groups grid-stride, accumulating w*value in FP64 in a fixed order;
-// each wave folds its lanes with a fixed butterfly, and the block's four wave
-// sums are added in wave order into one partial per block.  k_pf_final: one
-// block adds the partials in a fixed order.  The grid depends only on n, so
-// the sums are bit-reproducible for a given book.  The partials live in
-// stream-ordered memory (hipMallocAsync/hipFreeAsync), so calls on different
-// streams never share them.
+// Three kernels on the caller's stream.  k_pf_waves has the map's shape: one
+// 2-option group per work-item (64-thread blocks for prices, 256 for Greeks,
+// as the maps); each work-item forms w*value in FP64 and each wave folds its
+// lanes with a fixed butterfly into one partial.  k_pf_chunks: each block adds
+// a contiguous chunk of 2048 partials (eight per work-item, in order, then a
+// fixed tree).  k_pf_final: one block adds the chunk sums the same way.  All
+// grids depend only on n, so the sums are bit-reproducible for a given book.
+// The partials live in stream-ordered memory (hipMallocAsync/hipFreeAsync), so
+// calls on different streams never share them.
 //
 // Traffic: 21 bytes per option read (25 with weights), 8 or 40 bytes written
 // in all; the price form is bound by HBM reads and VALU together, the Greeks
@@ -27,9 +28,9 @@
 namespace bs {
 namespace {
 
-constexpr int kPfThreads = 256;
-constexpr int64_t kPfMaxBlocks = 16384;
-constexpr int kPfFinalThreads = 256;
+constexpr int kPfChunkThreads = 256;
+constexpr int kPfChunkPer = 8;                                  // partials per work-item
+constexpr int64_t kPfChunk = kPfChunkThreads * kPfChunkPer;     // partials per chunk block
 
 struct PfArgs {
     SoaIn<float> in;
@@ -55,44 +56,20 @@ __device__ __forceinline__ void add_option(double* acc, float s, float k, float 
     }
 }
 
-template <int NQ>
-__device__ __forceinline__ void block_sum(double* acc, double* out) {
-    __shared__ double lds[kPfThreads / 64][NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double x = acc[q];
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
-        acc[q] = x;
-    }
-    const int wave = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) lds[wave][q] = acc[q];
-    __syncthreads();
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double x = lds[0][q];
-            for (int wv = 1; wv < kPfThreads / 64; ++wv) x += lds[wv][q];
-            out[q] = x;
-        }
-}
-
-// VW = 2: aligned arrays, one 2-option group per step; VW = 1: any alignment.
-template <int VW, bool kGreeks>
-__global__ __launch_bounds__(kPfThreads) __attribute__((amdgpu_waves_per_eu(kGreeks ? 6 : 8)))
-void k_pf_blocks(PfArgs a) {
+// VW = 2: aligned arrays, one 2-option group per work-item; VW = 1: any
+// alignment, one option per work-item.
+template <int VW, bool kGreeks, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kGreeks ? 6 : 8)))
+void k_pf_waves(PfArgs a) {
     constexpr int NQ = kGreeks ? 5 : 1;
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * kPfThreads;
-    const int64_t gid = (int64_t)blockIdx.x * kPfThreads + threadIdx.x;
-    const int64_t ng = a.n / VW;
+    const int64_t gid = (int64_t)blockIdx.x * NT + threadIdx.x;
     if constexpr (VW == 2) {
         using F = vec<float, 2>;
-        for (int64_t g = gid; g < ng; g += stride) {
+        if (gid < a.n / 2) {
+            const int64_t g = gid;
             const F s = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.s) + g);
             const F k = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.k) + g);
             const F r = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.r) + g);
@@ -103,58 +80,95 @@ void k_pf_blocks(PfArgs a) {
             if (a.w) w = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.w) + g);
             add_option<kGreeks>(acc, s[0], k[0], r[0], v[0], t[0], put_of(ty, 0), w[0]);
             add_option<kGreeks>(acc, s[1], k[1], r[1], v[1], t[1], put_of(ty, 1), w[1]);
-        }
-        if (gid == 0 && (a.n & 1)) {   // the last option of an odd book
+        } else if (gid == a.n / 2 && (a.n & 1)) {   // the last option of an odd book
             const int64_t i = a.n - 1;
             add_option<kGreeks>(acc, a.in.s[i], a.in.k[i], a.in.r[i], a.in.v[i], a.in.t[i],
                                 a.type[i] != 0, a.w ? a.w[i] : 1.0f);
         }
     } else {
-        for (int64_t i = gid; i < a.n; i += stride)
+        if (gid < a.n) {
+            const int64_t i = gid;
             add_option<kGreeks>(acc, a.in.s[i], a.in.k[i], a.in.r[i], a.in.v[i], a.in.t[i],
                                 a.type[i] != 0, a.w ? a.w[i] : 1.0f);
+        }
     }
-    block_sum<NQ>(acc, a.partial + (int64_t)blockIdx.x * NQ);
+    const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x = acc[q];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+        if ((threadIdx.x & 63) == 0) a.partial[wave * NQ + q] = x;
+    }
 }
 
-// One block: thread t adds partials t, t + 256, ... in order, then a fixed tree.
+// Block b sums partials [b*kPfChunk, (b+1)*kPfChunk) (clipped to count): each
+// work-item adds its eight (stride kPfChunkThreads, all loads issued first) in
+// order, then a fixed tree over the block; out[b] is the chunk's sum.
 template <int NQ>
-__global__ __launch_bounds__(kPfFinalThreads) void k_pf_final(const double* __restrict__ partial,
-                                                              int64_t nblocks,
-                                                              double* __restrict__ sums) {
-    __shared__ double lds[kPfFinalThreads][NQ];
-    double acc[NQ];
+__global__ __launch_bounds__(kPfChunkThreads) void k_pf_chunks(const double* __restrict__ in,
+                                                               int64_t count,
+                                                               double* __restrict__ out) {
+    __shared__ double lds[NQ][kPfChunkThreads];
+    const int64_t base = (int64_t)blockIdx.x * kPfChunk + threadIdx.x;
+    double v[kPfChunkPer][NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int64_t b = threadIdx.x; b < nblocks; b += kPfFinalThreads)
+    for (int j = 0; j < kPfChunkPer; ++j) {
+        const int64_t i = base + (int64_t)j * kPfChunkThreads;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += partial[b * NQ + q];
+        for (int q = 0; q < NQ; ++q) v[j][q] = i < count ? in[i * NQ + q] : 0.0;
+    }
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) lds[threadIdx.x][q] = acc[q];
+    for (int q = 0; q < NQ; ++q) {
+        double acc = v[0][q];
+#pragma unroll
+        for (int j = 1; j < kPfChunkPer; ++j) acc += v[j][q];
+        lds[q][threadIdx.x] = acc;
+    }
     __syncthreads();
-    for (int half = kPfFinalThreads / 2; half >= 1; half >>= 1) {
+    for (int half = kPfChunkThreads / 2; half >= 1; half >>= 1) {
         if (threadIdx.x < half)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) lds[threadIdx.x][q] += lds[threadIdx.x + half][q];
+            for (int q = 0; q < NQ; ++q) lds[q][threadIdx.x] += lds[q][threadIdx.x + half];
         __syncthreads();
     }
-    if (threadIdx.x < NQ) sums[threadIdx.x] = lds[0][threadIdx.x];
+    if (threadIdx.x < NQ) out[(int64_t)blockIdx.x * NQ + threadIdx.x] = lds[threadIdx.x][0];
+}
+
+// Reduces count partials (count >= 1) to out[0..NQ) in a fixed order: chunk
+// sums until one is left.  tmp holds at least ceil(count / kPfChunk) entries.
+template <int NQ>
+void reduce_partials(const double* in, int64_t count, double* tmp, double* out, hipStream_t st) {
+    while (count > kPfChunk) {
+        const int64_t nb = (count + kPfChunk - 1) / kPfChunk;
+        hipLaunchKernelGGL((k_pf_chunks<NQ>), dim3((unsigned)nb), dim3(kPfChunkThreads), 0, st,
+                           in, count, tmp);
+        in = tmp;
+        tmp += nb * NQ;
+        count = nb;
+    }
+    hipLaunchKernelGGL((k_pf_chunks<NQ>), dim3(1), dim3(kPfChunkThreads), 0, st, in, count, out);
 }
 
 template <int VW, bool kGreeks>
 hipError_t launch_pf(const PfArgs& base, double* sums, hipStream_t st) {
     constexpr int NQ = kGreeks ? 5 : 1;
-    const int64_t items = VW == 2 ? base.n / 2 : base.n;
-    int64_t nb = (items + kPfThreads - 1) / kPfThreads;
-    if (nb > kPfMaxBlocks) nb = kPfMaxBlocks;
+    constexpr int NT = kGreeks ? 256 : 64;
+    // work-items: one per group, plus one for the last option of an odd book
+    const int64_t items = VW == 2 ? base.n / 2 + (base.n & 1) : base.n;
+    int64_t nb = (items + NT - 1) / NT;
     if (nb < 1) nb = 1;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    const int64_t nwaves = nb * (NT / 64);
+    // partials, then room for every level of chunk sums (each level is at most
+    // 1/kPfChunk of the one before)
+    const int64_t room = nwaves + nwaves / kPfChunk + 16;
     PfArgs a = base;
     hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&a.partial),
-                                  (size_t)nb * NQ * sizeof(double), st);
+                                  (size_t)room * NQ * sizeof(double), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_pf_blocks<VW, kGreeks>), dim3((unsigned)nb), dim3(kPfThreads), 0, st, a);
-    hipLaunchKernelGGL((k_pf_final<NQ>), dim3(1), dim3(kPfFinalThreads), 0, st, a.partial, nb,
-                       sums);
+    hipLaunchKernelGGL((k_pf_waves<VW, kGreeks, NT>), dim3((unsigned)nb), dim3(NT), 0, st, a);
+    reduce_partials<NQ>(a.partial, nwaves, a.partial + nwaves * NQ, sums, st);
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(a.partial, st);
     return e != hipSuccess ? e : f;
