@@ -9,6 +9,8 @@
 //   int bs_thread(void* tid_ptr)                        blackscholes.c:607-658
 //   struct map : ff_Map<int> { svc: NUM_RUNS x parallel_for(0, N, ...) }
 //                                                       blackscholes.c:315-348
+//   ff::ParallelFor::parallel_for / parallel_for_thid   fastflow/ff/parallel_for.hpp:111-317
+//   skepu2::Vector<T>(T*, size_t, bool), skepu2 Map<7>  blackscholes_skepu_omp.cpp:710-728
 // Every call below runs on the GPU through the C ABI (include/bs_mi355.h);
 // nothing here prices on the host.  Errors throw bs_compat::Error (the C ABI
 // itself never throws).
@@ -17,6 +19,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "bs_mi355.h"
 
@@ -130,4 +133,234 @@ class Map {
     void* dout_ = nullptr;
 };
 
+// ---- FastFlow ParallelFor front --------------------------------------------
+// The loop body the FastFlow driver hands to parallel_for_thid
+// (blackscholes.c:322-330): prices[i] = BlkSchlsEqEuroNoDiv(...) for every i of
+// the range, over the caller's host arrays.  The body's arrays stay resident on
+// the device between calls; refresh() re-uploads them after the host changed
+// an input.
+class PriceBody {
+   public:
+    PriceBody(int numOptions, const float* sptprice, const float* strike, const float* rate,
+              const float* volatility, const float* otime, const int* otype, float* prices)
+        : n_(numOptions), prices_(prices) {
+        h_[0] = sptprice;
+        h_[1] = strike;
+        h_[2] = rate;
+        h_[3] = volatility;
+        h_[4] = otime;
+        hty_ = otype;
+        check(bs_stream_create(&stream_), "PriceBody: stream");
+        const size_t fb = (size_t)(n_ > 0 ? n_ : 1) * sizeof(float);
+        for (void*& p : d_) check(bs_malloc(&p, fb), "PriceBody: alloc");
+        check(bs_malloc(&dty_, fb), "PriceBody: alloc");
+        check(bs_malloc(&dout_, fb), "PriceBody: alloc");
+        refresh();
+    }
+    ~PriceBody() {
+        if (stream_) bs_stream_sync(stream_);
+        for (void* p : d_) bs_free(p);
+        bs_free(dty_);
+        bs_free(dout_);
+        if (stream_) bs_stream_destroy(stream_);
+    }
+    PriceBody(const PriceBody&) = delete;
+    PriceBody& operator=(const PriceBody&) = delete;
+
+    void refresh() {
+        const size_t fb = (size_t)n_ * sizeof(float);
+        if (fb == 0) return;
+        for (int i = 0; i < 5; ++i) check(bs_memcpy_h2d(d_[i], h_[i], fb, stream_), "PriceBody: upload");
+        check(bs_memcpy_h2d(dty_, hty_, fb, stream_), "PriceBody: upload");
+    }
+    // Prices [first, last) on the GPU and copies those prices to the host.
+    void run_range(long first, long last) {
+        if (first < 0 || last > n_ || first > last) throw Error(BS_ERR_INVALID, "PriceBody: range");
+        const int64_t m = last - first;
+        if (m == 0) return;
+        float* f[5];
+        for (int i = 0; i < 5; ++i) f[i] = static_cast<float*>(d_[i]) + first;
+        check(bs_price_i32type_f32(f[0], f[1], f[2], f[3], f[4], static_cast<int32_t*>(dty_) + first,
+                                   static_cast<float*>(dout_) + first, m, stream_),
+              "PriceBody: price");
+        check(bs_memcpy_d2h(prices_ + first, static_cast<float*>(dout_) + first,
+                            (size_t)m * sizeof(float), stream_),
+              "PriceBody: download");
+        check(bs_stream_sync(stream_), "PriceBody: sync");
+    }
+
+   private:
+    int n_;
+    float* prices_;
+    const float* h_[5];
+    const int* hty_;
+    bs_stream_t stream_ = nullptr;
+    void* d_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* dty_ = nullptr;
+    void* dout_ = nullptr;
+};
+
+// ff::ParallelFor's interface for the Black-Scholes loop: the same
+// constructor and parallel_for / parallel_for_thid / parallel_for_static
+// signatures; the body must be a PriceBody (any other body is a compile-time
+// error: nothing runs on the host).  The iteration space is priced in one GPU
+// launch, so grain, nw, spinwait and the worker count have nothing to set;
+// step must be 1 (the reference's loops use 1).
+class ParallelFor {
+   public:
+    explicit ParallelFor(long maxnw = -1, bool spinwait = false, bool spinbarrier = false) {
+        (void)maxnw;
+        (void)spinwait;
+        (void)spinbarrier;
+    }
+    template <class F>
+    void parallel_for(long first, long last, const F& f, long nw = -1) {
+        parallel_for(first, last, 1, 0, f, nw);
+    }
+    template <class F>
+    void parallel_for(long first, long last, long step, const F& f, long nw = -1) {
+        parallel_for(first, last, step, 0, f, nw);
+    }
+    template <class F>
+    void parallel_for(long first, long last, long step, long grain, const F& f, long nw = -1) {
+        static_assert(std::is_same<F, PriceBody>::value,
+                      "bs_compat::ParallelFor runs the Black-Scholes body (PriceBody) only");
+        (void)grain;
+        (void)nw;
+        if (step != 1) throw Error(BS_ERR_INVALID, "ParallelFor: step must be 1");
+        const_cast<F&>(f).run_range(first, last);
+    }
+    template <class F>
+    void parallel_for_thid(long first, long last, long step, long grain, const F& f, long nw = -1) {
+        parallel_for(first, last, step, grain, f, nw);
+    }
+    template <class F>
+    void parallel_for_static(long first, long last, long step, long grain, const F& f,
+                             long nw = -1) {
+        parallel_for(first, last, step, grain, f, nw);
+    }
+};
+
+// ---- SkePU2 Vector / Map front ------------------------------------------------
+namespace skepu {
+
+// skepu2::BackendSpec{Backend::Type::...}: the GPU types are accepted; the
+// CPU and OpenMP types are refused, since this library never prices on the
+// host.
+struct Backend {
+    enum class Type { CPU, OpenMP, CUDA, OpenCL };
+};
+struct BackendSpec {
+    Backend::Type type;
+    explicit BackendSpec(Backend::Type t) : type(t) {}
+    void setCPUThreads(size_t) {}
+    void setGPUThreads(size_t) {}
+    void setGPUBlocks(size_t) {}
+    void setDevices(size_t) {}
+};
+
+// skepu2::Vector<T>(T* p, size_t n, bool dealloc): wraps caller memory with a
+// lazily synchronised device copy, as SkePU's CUDA back-end does
+// (map_cu.inl:15-58): a map uploads an input only when the host copy is newer
+// and marks its output's device copy newer; host reads download it again only
+// then.  Non-const host access invalidates the device copy.
+template <class T>
+class Vector {
+   public:
+    Vector(T* p, size_t n, bool dealloc = false) : h_(p), n_(n), own_(dealloc) {}
+    explicit Vector(size_t n) : h_(new T[n]()), n_(n), own_(true) {}
+    ~Vector() {
+        if (d_) bs_free(d_);
+        if (own_) delete[] h_;
+    }
+    Vector(const Vector&) = delete;
+    Vector& operator=(const Vector&) = delete;
+
+    size_t size() const { return n_; }
+    T& operator[](size_t i) {
+        updateHost();
+        invalidateDeviceData();
+        return h_[i];
+    }
+    const T& operator[](size_t i) const {
+        updateHost();
+        return h_[i];
+    }
+    T* data() {
+        updateHost();
+        invalidateDeviceData();
+        return h_;
+    }
+    void flush() const { updateHost(); }
+    void updateHost() const {
+        if (host_valid_) return;
+        check(bs_memcpy_d2h(h_, d_, n_ * sizeof(T), nullptr), "Vector::updateHost");
+        check(bs_stream_sync(nullptr), "Vector::updateHost");
+        host_valid_ = true;
+        ++downloads_;
+    }
+    void invalidateDeviceData() { device_valid_ = false; }
+
+    // Device side (used by Map): read access uploads a stale device copy;
+    // write access marks the device copy as the newer one.
+    T* device_read() {
+        alloc();
+        if (!device_valid_ && n_ > 0) {
+            check(bs_memcpy_h2d(d_, h_, n_ * sizeof(T), nullptr), "Vector::updateDevice");
+            ++uploads_;
+        }
+        device_valid_ = true;
+        return static_cast<T*>(d_);
+    }
+    T* device_write() {
+        alloc();
+        device_valid_ = true;
+        host_valid_ = false;
+        return static_cast<T*>(d_);
+    }
+    int uploads() const { return uploads_; }
+    int downloads() const { return downloads_; }
+
+   private:
+    void alloc() {
+        if (!d_) check(bs_malloc(&d_, (n_ > 0 ? n_ : 1) * sizeof(T)), "Vector: alloc");
+    }
+    T* h_;
+    size_t n_;
+    bool own_;
+    void* d_ = nullptr;
+    mutable bool host_valid_ = true;
+    bool device_valid_ = false;
+    mutable int downloads_ = 0;
+    int uploads_ = 0;
+};
+
+// The reference's skepu2 Map<7> over the Black-Scholes user function
+// (blackscholes_skepu_omp.cpp:718-728): map(prices, sptprice, strike, rate,
+// volatility, otime, otype, data).  The OptionData vector is accepted and not
+// read, as the user function does not read it for pricing.
+class Map {
+   public:
+    void setBackend(const BackendSpec& spec) {
+        if (spec.type == Backend::Type::CPU || spec.type == Backend::Type::OpenMP)
+            throw Error(BS_ERR_INVALID, "skepu::Map: only GPU back-ends are available");
+    }
+    template <class Any>
+    void operator()(Vector<float>& prices, Vector<float>& sptprice, Vector<float>& strike,
+                    Vector<float>& rate, Vector<float>& volatility, Vector<float>& otime,
+                    Vector<int>& otype, Vector<Any>& /*data*/) {
+        const size_t n = prices.size();
+        if (sptprice.size() != n || strike.size() != n || rate.size() != n ||
+            volatility.size() != n || otime.size() != n || otype.size() != n)
+            throw Error(BS_ERR_INVALID, "skepu::Map: size mismatch");
+        if (n == 0) return;
+        check(bs_price_i32type_f32(sptprice.device_read(), strike.device_read(), rate.device_read(),
+                                   volatility.device_read(), otime.device_read(),
+                                   otype.device_read(), prices.device_write(), (int64_t)n,
+                                   nullptr),
+              "skepu::Map");
+    }
+};
+
+}  // namespace skepu
 }  // namespace bs_compat

@@ -73,3 +73,22 @@ def test_compat_header_forms(gpu, oracle, tmp_path):
     assert np.max(np.abs(cols[:64, 3] - ref[:64])) <= 1e-4
     x = (np.arange(64) - 32) / 8.0
     assert np.max(np.abs(cols[:64, 4] - oracle.cndf_f32(x.astype(np.float32)))) < 3e-7
+    for j in (5, 6):                                   # ParallelFor, SkePU Map
+        assert np.max(np.abs(cols[:, j] - ref)) <= 1e-4
+        assert np.array_equal(cols[:, j].astype(np.float32), cols[:, 0].astype(np.float32))
+    # SkePU lazy sync: 100 map calls upload each input once and download the
+    # prices once (at flush); the OpenMP back-end is refused (no host pricing)
+    tail = open(out).read().strip().splitlines()[-1].split()
+    assert tail[0] == "#" and tail[2] == "1" and tail[4] == "1" and tail[6] == "1", tail
+
+
+def test_parallel_for_refuses_host_bodies(tmp_path):
+    """ParallelFor runs the Black-Scholes body only: any other loop body is a
+    compile-time error, so no loop silently runs on the host."""
+    src = tmp_path / "bad.cpp"
+    src.write_text('#include "bs_compat.hpp"\n'
+                   "int main() { bs_compat::ParallelFor pf;\n"
+                   "  pf.parallel_for(0, 10, 1, 0, [](long, int) {}); return 0; }\n")
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert p.returncode != 0 and "Black-Scholes body" in p.stderr
