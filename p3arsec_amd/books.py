@@ -94,6 +94,34 @@ def edge_book() -> Book:
     return Book(*cols, otype)
 
 
+def degenerate_book() -> Book:
+    """Inputs outside the distributions, where the reference's arithmetic
+    meets IEEE special values: T = 0 and sigma = 0 (division by zero, d =
+    +-inf), S = K with T = 0 (0/0), S = 0 and K = 0 (log of 0 and of inf),
+    negative T (square root of a negative), zero and negative rates, very
+    large and very small magnitudes, and NaN and infinite inputs; every row
+    as call and as put."""
+    inf, nan = float("inf"), float("nan")
+    rows = [
+        (100.0, 90.0, 0.05, 0.3, 0.0), (90.0, 100.0, 0.05, 0.3, 0.0), (100.0, 100.0, 0.05, 0.3, 0.0),
+        (100.0, 90.0, 0.05, 0.0, 1.0), (90.0, 100.0, 0.05, 0.0, 1.0), (100.0, 100.0, 0.0, 0.0, 1.0),
+        (100.0, 90.0, 0.05, 0.0, 0.0), (0.0, 100.0, 0.05, 0.3, 1.0), (100.0, 0.0, 0.05, 0.3, 1.0),
+        (0.0, 0.0, 0.05, 0.3, 1.0), (100.0, 90.0, 0.05, 0.3, -1.0), (100.0, 90.0, 0.0, 0.3, 1.0),
+        (100.0, 90.0, -0.05, 0.3, 1.0), (1e30, 1e30, 0.05, 0.3, 1.0), (1e-30, 1e-30, 0.05, 0.3, 1.0),
+        (100.0, 90.0, 0.05, 1e-30, 1.0), (100.0, 90.0, 0.05, 1e10, 1.0), (100.0, 90.0, 0.05, 0.3, 1e-30),
+        (100.0, 90.0, 0.05, 0.3, 1e10), (100.0, 90.0, 50.0, 0.3, 1.0), (inf, 100.0, 0.05, 0.3, 1.0),
+        (100.0, inf, 0.05, 0.3, 1.0), (100.0, 90.0, inf, 0.3, 1.0), (100.0, 90.0, 0.05, inf, 1.0),
+        (100.0, 90.0, 0.05, 0.3, inf), (nan, 100.0, 0.05, 0.3, 1.0), (100.0, nan, 0.05, 0.3, 1.0),
+        (100.0, 90.0, nan, 0.3, 1.0), (100.0, 90.0, 0.05, nan, 1.0), (100.0, 90.0, 0.05, 0.3, nan),
+        (-100.0, 90.0, 0.05, 0.3, 1.0), (100.0, -90.0, 0.05, 0.3, 1.0),
+    ]
+    a = np.array(rows, dtype=np.float64)
+    n = a.shape[0]
+    cols = [np.concatenate([a[:, j], a[:, j]]).astype(np.float32) for j in range(5)]
+    otype = np.concatenate([np.zeros(n, np.uint8), np.ones(n, np.uint8)])
+    return Book(*cols, otype)
+
+
 # ---------------------------------------------------------------------------
 # PARSEC text format (blackscholes.c:701-734 read, :923-947 write).
 def write_parsec_input(path: str, book: Book) -> None:

@@ -68,18 +68,23 @@ constexpr SplitConst split(double c) {
 __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // a / b: quotient from the hardware reciprocal, then one residual step
-// (the residual a - b q is exact in an FMA).
+// (the residual a - b q is exact in an FMA).  Where the residual is not a
+// number (b = 0 or +-inf, a = +-inf) the unrefined quotient already is IEEE's
+// answer (+-inf, 0 or NaN), so it is returned: an option with T = 0 or
+// sigma = 0 divides by zero in the reference too and prices from d = +-inf.
 __device__ __forceinline__ float div_rn(float a, float b) {
     const float r = hw_rcp(b);
     const float q = a * r;
     const float e = __builtin_fmaf(-b, q, a);
-    return __builtin_fmaf(e, r, q);
+    const float x = __builtin_fmaf(e, r, q);
+    return x == x ? x : q;
 }
 
 __device__ __forceinline__ float recip_rn(float b) {
     const float r = hw_rcp(b);
     const float e = __builtin_fmaf(-b, r, 1.0f);
-    return __builtin_fmaf(e, r, r);
+    const float x = __builtin_fmaf(e, r, r);
+    return x == x ? x : r;
 }
 
 // sqrt(x): hardware root, then pick among it and its two neighbours by the
@@ -139,7 +144,12 @@ struct Ref32 {
                              b2 = split(kB2), b3 = split(kB3), b4 = split(kB4),
                              b5 = split(kB5);
         const bool neg = x < 0.0f;
-        const float ax = __builtin_fabsf(x);   // |-0| = +0 gives the same bits below
+        // |x| capped at 1e10: for every |x| >= 20 the reference returns exactly
+        // 1 (e^{-x^2/2} is 0 or denormal and the tail rounds away), and so does
+        // the capped value; uncapped, |x| = inf would meet inf*hi + inf*lo = NaN
+        // in mul_dc where the reference's double product is inf.  NaN passes.
+        float ax = __builtin_fabsf(x);   // |-0| = +0 gives the same bits below
+        ax = ax > 1e10f ? 1e10f : ax;
         float n = exp(-0.5f * ax * ax);
         n = mul_dc(n, c0);
         float k = mul_dc(ax, cp);

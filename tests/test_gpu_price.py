@@ -59,6 +59,24 @@ def test_golden_vectors(gpu, oracle, tag, mode):
     assert mx <= (TOL_FAST if mode == "fast" else TOL)
 
 
+@pytest.mark.parametrize("mode", ["ref", "i32"])
+def test_degenerate_inputs_match_reference(gpu, mode):
+    """Options outside the distributions (T = 0, sigma = 0, S or K zero or
+    infinite, 0/0, negative T, NaN inputs, magnitudes near 1e30 and 1e-30):
+    NaN exactly where the reference gives NaN, the same infinities, and every
+    finite price within 1e-4 of the reference's or, for prices above 1e2,
+    within 1e-6 relative (eight float ulps)."""
+    book, ref = load_golden("degenerate")
+    got = run(gpu, book, mode)
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(got), nan)
+    inf = np.isinf(ref)
+    assert np.array_equal(got[inf], ref[inf])
+    fin = ~nan & ~inf
+    d = np.abs(got[fin].astype(np.float64) - ref[fin].astype(np.float64))
+    assert np.all(d <= np.maximum(1e-4, 1e-6 * np.abs(ref[fin].astype(np.float64))))
+
+
 def test_random_book_1m(gpu, oracle):
     book = books.make_book(1 << 20, books.SEED_BASE + 21)
     ref, _, _ = oracle.price_f32(book)
@@ -146,6 +164,15 @@ def test_cndf_batch(gpu, oracle):
     got = out.download(np.float32, x.size)
     ref = oracle.cndf_f32(x.astype(np.float32))
     assert np.max(np.abs(got.astype(np.float64) - ref)) < 3e-7
+    # special values: the reference's CNDF gives 1 at +inf, 0 at -inf, NaN at NaN
+    sp = np.array([np.inf, -np.inf, np.nan, 1e30, -1e30, 25.0, -25.0], np.float32)
+    ds = gpu.DeviceBuffer.from_array(sp)
+    os_ = gpu.DeviceBuffer(sp.size * 4)
+    gpu.call("bs_cndf_f32", ds.ptr, os_.ptr, sp.size, None)
+    gpu.call("bs_stream_sync", None)
+    gs, rs = os_.download(np.float32, sp.size), oracle.cndf_f32(sp)
+    assert np.array_equal(np.isnan(gs), np.isnan(rs)) and np.isnan(gs).sum() == 1
+    assert np.array_equal(gs[~np.isnan(gs)], rs[~np.isnan(rs)].astype(np.float32))
     d64 = gpu.DeviceBuffer.from_array(x)
     o64 = gpu.DeviceBuffer(x.size * 8)
     gpu.call("bs_cndf_f64", d64.ptr, o64.ptr, x.size, None)

@@ -24,6 +24,20 @@ def test_golden_bit_exact(oracle):
         assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32)), tag
 
 
+def test_golden_degenerate_inputs(oracle):
+    """T = 0, sigma = 0, zero or infinite S and K, NaN inputs...: the oracle
+    gives the compiled reference's output for every option: NaN where it
+    prints NaN, and otherwise the same "%.18f" text, infinities included (the
+    fixture holds the reference's printed values, so a price below 5e-19
+    reads back as 0)."""
+    book, ref = load_golden("degenerate")
+    mine, _, _ = oracle.price_f32(book, nthreads=1)
+    nan = np.isnan(ref)
+    assert 0 < nan.sum() < book.n and np.isinf(ref).sum() > 0 and np.isfinite(ref).sum() > 20
+    assert np.array_equal(np.isnan(mine), nan)
+    assert ["%.18f" % x for x in mine[~nan]] == ["%.18f" % x for x in ref[~nan]]
+
+
 def test_cndf_error_bound_against_exact_phi(oracle):
     x = np.linspace(-8.0, 8.0, 200001)
     err64 = np.abs(oracle.cndf_f64(x) - ndtr(x))
