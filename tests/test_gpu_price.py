@@ -77,6 +77,19 @@ def test_degenerate_inputs_match_reference(gpu, mode):
     assert np.all(d <= np.maximum(1e-4, 1e-6 * np.abs(ref[fin].astype(np.float64))))
 
 
+def test_degenerate_inputs_fp64(gpu, oracle):
+    """fptype = double on the same degenerate book: NaN where the FP64
+    restatement gives NaN, and the same infinities and finite values."""
+    book, _ = load_golden("degenerate")
+    got = run(gpu, book.astype(np.float64), "f64")
+    ref = oracle.price_f64(book)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    fin = ok & np.isfinite(ref)
+    assert np.array_equal(got[ok & ~fin], ref[ok & ~fin])
+    assert np.all(np.abs(got[fin] - ref[fin]) <= 1e-9 * np.maximum(1.0, np.abs(ref[fin])))
+
+
 def test_random_book_1m(gpu, oracle):
     book = books.make_book(1 << 20, books.SEED_BASE + 21)
     ref, _, _ = oracle.price_f32(book)
