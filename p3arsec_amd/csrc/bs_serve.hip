@@ -175,7 +175,8 @@ struct bs_ctx_s {
     std::condition_variable cv_done;   // clients: ticket state changes
     bs::serve::Core* core = nullptr;
     char* arena = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // capture stream; batches alternate over streams[]
+    hipStream_t streams[2] = {nullptr, nullptr};
     uint8_t* host_slot[bs::kSlots] = {};      // pinned, written by the dispatcher
     uint8_t* host_slot_dev[bs::kSlots] = {};  // the same memory, device address
     uint8_t* dev_slot[bs::kSlots] = {};
@@ -210,20 +211,21 @@ static_assert(serve::kErrBudget == BS_ERR_BUDGET && serve::kErrBusy == BS_ERR_BU
                   serve::kCancelled == BS_TICKET_CANCELLED && serve::kGreeks == BS_REQ_GREEKS,
               "serve_core codes mirror bs_mi355.h");
 
-// Launches one slot: graph replay, or the stage kernel and the fused kernel of
-// each non-empty section directly, each section's grid one block per tile.
-hipError_t launch_slot(bs_ctx_s* c, int s) {
-    if (c->graph[s]) return hipGraphLaunch(c->graph[s], c->stream);
+// Launches one slot on stream st: graph replay, or the stage kernel and the
+// fused kernel of each non-empty section directly, each section's grid one
+// block per tile.
+hipError_t launch_slot(bs_ctx_s* c, int s, hipStream_t st) {
+    if (c->graph[s]) return hipGraphLaunch(c->graph[s], st);
     const SlotHeader* h = reinterpret_cast<const SlotHeader*>(c->host_slot[s]);
-    hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, c->stream,
+    hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, st,
                        reinterpret_cast<const uint4*>(c->host_slot_dev[s]),
                        reinterpret_cast<uint4*>(c->dev_slot[s]));
     if (h->tiles[0] > 0)
         hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3((unsigned)std::min(h->tiles[0], kMaxGrid)),
-                           dim3(kThreads), 0, c->stream, c->dev_slot[s]);
+                           dim3(kThreads), 0, st, c->dev_slot[s]);
     if (h->tiles[1] > 0)
         hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3((unsigned)std::min(h->tiles[1], kMaxGrid)),
-                           dim3(kThreads), 0, c->stream, c->dev_slot[s]);
+                           dim3(kThreads), 0, st, c->dev_slot[s]);
     return hipGetLastError();
 }
 
@@ -281,19 +283,24 @@ void write_slot(bs_ctx_s* c, int s, const std::vector<serve::Admitted>& reqs) {
 
 void dispatcher(bs_ctx_s* c) {
     (void)hipSetDevice(c->device);
-    std::vector<Batch> inflight;        // launch order; owned by this thread
+    std::vector<Batch> inflight;        // batches on the GPU; owned by this thread
     bool busy[kSlots] = {};
     int64_t blocked_gen = -1;           // gen at the last admission that found no room
     int64_t t_empty = -1;               // when the last batch in flight retired
+    int64_t nlaunch = 0;                // batches alternate over the two streams
     std::vector<serve::Admitted> reqs;  // reused: no per-batch allocation
     reqs.reserve(kMaxDesc);
     for (;;) {
         bool progressed = false;
-        // retire finished batches in launch order
-        while (!inflight.empty()) {
-            Batch& b = inflight.front();
+        // retire every finished batch (batches on the two streams may finish
+        // out of launch order; each is independent of the others)
+        for (size_t bi = 0; bi < inflight.size();) {
+            Batch& b = inflight[bi];
             const hipError_t q = hipEventQuery(b.done);
-            if (q == hipErrorNotReady) break;
+            if (q == hipErrorNotReady) {
+                ++bi;
+                continue;
+            }
             const int64_t t = now_ns();
             {
                 std::lock_guard<std::mutex> lk(c->mu);
@@ -305,7 +312,7 @@ void dispatcher(bs_ctx_s* c) {
             }
             c->cv_done.notify_all();
             busy[b.slot] = false;
-            inflight.erase(inflight.begin());
+            inflight.erase(inflight.begin() + (std::ptrdiff_t)bi);
             const int64_t t2 = now_ns();
             c->t_retire.fetch_add(t2 - t, std::memory_order_relaxed);
             if (inflight.empty()) t_empty = t2;
@@ -339,8 +346,9 @@ void dispatcher(bs_ctx_s* c) {
                     c->t_idle.fetch_add(b.t_launch - t_empty, std::memory_order_relaxed);
                     c->idle_launches.fetch_add(1, std::memory_order_relaxed);
                 }
-                hipError_t e = launch_slot(c, s);
-                if (e == hipSuccess) e = hipEventRecord(c->ev[s], c->stream);
+                hipStream_t st = c->streams[nlaunch++ & 1];
+                hipError_t e = launch_slot(c, s, st);
+                if (e == hipSuccess) e = hipEventRecord(c->ev[s], st);
                 c->t_launch.fetch_add(now_ns() - b.t_launch, std::memory_order_relaxed);
                 if (e != hipSuccess) {
                     {
@@ -409,6 +417,8 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipMalloc((void**)&c->arena, hbm_budget_bytes);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int j = 0; j < 2 && e == hipSuccess; ++j)
+        e = hipStreamCreateWithFlags(&c->streams[j], hipStreamNonBlocking);
     for (int s = 0; s < kSlots && e == hipSuccess; ++s) {
         e = hipHostMalloc((void**)&c->host_slot[s], kSlotBytes, hipHostMallocCoherent);
         if (e == hipSuccess)
@@ -452,6 +462,8 @@ int bs_ctx_destroy(bs_ctx_t ctx) {
     }
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t st : c->streams)
+        if (st) (void)hipStreamSynchronize(st);
     for (int s = 0; s < kSlots; ++s) {
         if (c->graph[s]) (void)hipGraphExecDestroy(c->graph[s]);
         if (c->ev[s]) (void)hipEventDestroy(c->ev[s]);
@@ -459,6 +471,8 @@ int bs_ctx_destroy(bs_ctx_t ctx) {
         if (c->dev_slot[s]) (void)hipFree(c->dev_slot[s]);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    for (hipStream_t st : c->streams)
+        if (st) (void)hipStreamDestroy(st);
     if (c->arena) (void)hipFree(c->arena);
     delete c->core;
     delete c;
