@@ -24,18 +24,14 @@
 #include <time.h>
 
 #include <algorithm>
-#include <atomic>
-#include <chrono>
-#include <condition_variable>
-#include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "bs_internal.hpp"
 #include "bs_math.hpp"
 #include "bs_ops.hpp"
 #include "serve_core.hpp"
+#include "serve_loop.hpp"
 
 namespace bs {
 namespace {
@@ -45,15 +41,6 @@ constexpr int kTile = kThreads * kVW;  // options per tile: one block step
 constexpr int kMaxDesc = 4096;         // requests per batch: 64 x 64 for find_desc
 constexpr int64_t kMaxGrid = int64_t(1) << 20;  // direct-launch grid cap (blocks loop)
 constexpr int kStageBlocks = 32;
-// Batches in flight at once.  While one runs, the next ones are queued behind
-// it on the stream, so the GPU stays busy while clients turn finished
-// requests around into new ones.
-constexpr int kSlots = 4;
-constexpr int kReapSpinUs = 200;
-// With fewer than two batches on the GPU and more than this many options
-// queued, the dispatcher admits half of the queue so that the other half's
-// launch (and the clients' turnaround of the first half) overlaps the first.
-constexpr int64_t kSplitOptions = int64_t(4) << 20;
 
 // One request of a batch: 112 bytes, a multiple of the stage kernel's 16-byte words.
 struct Desc {
@@ -145,60 +132,170 @@ void k_serve(const uint8_t* __restrict__ slot) {
 using ServePrice = PriceF32<Ref32, kVW>;
 using ServeGreeks = GreeksF32<kVW>;
 
-int64_t now_ns() {
-    return std::chrono::duration_cast<std::chrono::nanoseconds>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
+}  // namespace
 
-struct Batch {
-    int slot;
-    hipEvent_t done;
-    int64_t t_launch;
-    std::vector<int64_t> tickets;
+// The HIP executor of the serving loop (serve_loop.hpp): descriptor tables,
+// the stage and fused kernels, two streams and one event per slot.
+struct HipExec {
+    // Batches in flight at once.  While one runs, the next ones are queued
+    // behind it, so the GPU stays busy while clients turn finished requests
+    // around into new ones.
+    static constexpr int kSlots = 4;
+
+    int device = 0;
+    int use_graph = 0;
+    int grid = 0;
+    char* arena = nullptr;
+    hipStream_t stream = nullptr;               // capture stream
+    hipStream_t streams[2] = {nullptr, nullptr};  // batches alternate over these
+    uint8_t* host_slot[kSlots] = {};            // pinned, written by the dispatcher
+    uint8_t* host_slot_dev[kSlots] = {};        // the same memory, device address
+    uint8_t* dev_slot[kSlots] = {};
+    hipGraphExec_t graph[kSlots] = {};
+    hipEvent_t ev[kSlots] = {};
+    int64_t seq = 0, nlaunch = 0;
+
+    hipError_t init(int dev, size_t arena_bytes, int graph_on, int grid_blocks) {
+        device = dev;
+        use_graph = graph_on;
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipMalloc((void**)&arena, arena_bytes);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        for (int j = 0; j < 2 && e == hipSuccess; ++j)
+            e = hipStreamCreateWithFlags(&streams[j], hipStreamNonBlocking);
+        for (int s = 0; s < kSlots && e == hipSuccess; ++s) {
+            e = hipHostMalloc((void**)&host_slot[s], kSlotBytes, hipHostMallocCoherent);
+            if (e == hipSuccess)
+                e = hipHostGetDevicePointer((void**)&host_slot_dev[s], host_slot[s], 0);
+            if (e == hipSuccess) e = hipMalloc((void**)&dev_slot[s], kSlotBytes);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[s], hipEventDisableTiming);
+            if (e == hipSuccess) ::memset(host_slot[s], 0, sizeof(SlotHeader));
+        }
+        if (e == hipSuccess) {
+            int cus = 0;
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+            grid = grid_blocks > 0 ? grid_blocks : std::max(1, cus) * 8;
+        }
+        for (int s = 0; s < kSlots && e == hipSuccess && use_graph; ++s) e = capture(s);
+        return e;
+    }
+    void release() {
+        (void)hipSetDevice(device);
+        drain();
+        for (int s = 0; s < kSlots; ++s) {
+            if (graph[s]) (void)hipGraphExecDestroy(graph[s]);
+            if (ev[s]) (void)hipEventDestroy(ev[s]);
+            if (host_slot[s]) (void)hipHostFree(host_slot[s]);
+            if (dev_slot[s]) (void)hipFree(dev_slot[s]);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+        for (hipStream_t st : streams)
+            if (st) (void)hipStreamDestroy(st);
+        if (arena) (void)hipFree(arena);
+    }
+
+    // Captures slot s as one graph: stage, price section, Greeks section,
+    // each fused kernel over the fixed persistent grid.
+    hipError_t capture(int s) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, stream,
+                           reinterpret_cast<const uint4*>(host_slot_dev[s]),
+                           reinterpret_cast<uint4*>(dev_slot[s]));
+        hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3(grid), dim3(kThreads), 0, stream,
+                           dev_slot[s]);
+        hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3(grid), dim3(kThreads), 0, stream,
+                           dev_slot[s]);
+        e = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess) return e;
+        e = hipGraphInstantiate(&graph[s], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+
+    // Fills slot s with the batch's descriptors, price requests first, each
+    // section in admission order.  The slot is the dispatcher's own while no
+    // batch uses it, so no lock is held.
+    void write_slot(int s, const std::vector<serve::Admitted>& reqs) {
+        SlotHeader* h = reinterpret_cast<SlotHeader*>(host_slot[s]);
+        Desc* d = reinterpret_cast<Desc*>(host_slot[s] + sizeof(SlotHeader));
+        int nd[2] = {0, 0};
+        for (const serve::Admitted& r : reqs) ++nd[r.kind == serve::kGreeks ? 1 : 0];
+        int next[2] = {0, nd[0]};
+        int64_t tiles[2] = {0, 0};
+        for (const serve::Admitted& r : reqs) {
+            const int sec = r.kind == serve::kGreeks ? 1 : 0;
+            size_t bo[6], oo[5];
+            serve::book_layout(r.book_n, bo);
+            serve::output_layout(r.n, r.kind, oo);
+            Desc& x = d[next[sec]++];
+            for (int j = 0; j < 5; ++j) x.in[j] = reinterpret_cast<const float*>(arena + r.book_off + bo[j]);
+            x.type = reinterpret_cast<const uint8_t*>(arena + r.book_off + bo[5]);
+            for (int j = 0; j < 5; ++j) x.out[j] = reinterpret_cast<float*>(arena + r.out_off + oo[j]);
+            x.n = r.n;
+            x.tile_begin = tiles[sec];
+            x.kind = r.kind;
+            x.pad = 0;
+            tiles[sec] += (r.n + kTile - 1) / kTile;
+        }
+        h->ndesc[0] = nd[0];
+        h->ndesc[1] = nd[1];
+        h->tiles[0] = tiles[0];
+        h->tiles[1] = tiles[1];
+        h->seq = ++seq;
+    }
+
+    // ---- the executor interface ----
+    void thread_start() { (void)hipSetDevice(device); }
+
+    // Launches slot s on the next stream: graph replay, or the stage kernel
+    // and the fused kernel of each non-empty section directly, each
+    // section's grid one block per tile; then records the slot's event.
+    int launch(int s, const std::vector<serve::Admitted>& reqs, std::string* err) {
+        write_slot(s, reqs);
+        hipStream_t st = streams[nlaunch++ & 1];
+        hipError_t e;
+        if (graph[s]) {
+            e = hipGraphLaunch(graph[s], st);
+        } else {
+            const SlotHeader* h = reinterpret_cast<const SlotHeader*>(host_slot[s]);
+            hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, st,
+                               reinterpret_cast<const uint4*>(host_slot_dev[s]),
+                               reinterpret_cast<uint4*>(dev_slot[s]));
+            if (h->tiles[0] > 0)
+                hipLaunchKernelGGL((k_serve<ServePrice, 0>),
+                                   dim3((unsigned)std::min(h->tiles[0], kMaxGrid)), dim3(kThreads),
+                                   0, st, dev_slot[s]);
+            if (h->tiles[1] > 0)
+                hipLaunchKernelGGL((k_serve<ServeGreeks, 1>),
+                                   dim3((unsigned)std::min(h->tiles[1], kMaxGrid)), dim3(kThreads),
+                                   0, st, dev_slot[s]);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev[s], st);
+        if (e != hipSuccess) *err = hipGetErrorString(e);
+        return e == hipSuccess ? 0 : -1;
+    }
+    int query(int s, std::string* err) {
+        const hipError_t q = hipEventQuery(ev[s]);
+        if (q == hipErrorNotReady) return 0;
+        if (q == hipSuccess) return 1;
+        *err = hipGetErrorString(q);
+        return -1;
+    }
+    void drain() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t st : streams)
+            if (st) (void)hipStreamSynchronize(st);
+    }
 };
 
-}  // namespace
 }  // namespace bs
 
-// Locking: `mu` guards `core`, `fatal` and `batch_ns`.  The dispatcher takes it
-// only to admit a batch or retire one; while batches run it polls their
-// events with the lock released, so submitting and reaping threads never wait
-// behind the poll loop.  `gen` counts the changes that can unblock admission
-// (a submit, a freed result or book, a cancel, shutdown); `queued` mirrors
-// the queue length.  Both are written under `mu` and read without it.
 struct bs_ctx_s {
-    int device = 0;
-    bs_serve_opts opts{};
-    std::mutex mu;
-    std::condition_variable cv_work;   // dispatcher: new requests / memory / shutdown
-    std::condition_variable cv_done;   // clients: ticket state changes
-    bs::serve::Core* core = nullptr;
-    char* arena = nullptr;
-    hipStream_t stream = nullptr;      // capture stream; batches alternate over streams[]
-    hipStream_t streams[2] = {nullptr, nullptr};
-    uint8_t* host_slot[bs::kSlots] = {};      // pinned, written by the dispatcher
-    uint8_t* host_slot_dev[bs::kSlots] = {};  // the same memory, device address
-    uint8_t* dev_slot[bs::kSlots] = {};
-    hipGraphExec_t graph[bs::kSlots] = {};
-    hipEvent_t ev[bs::kSlots] = {};
-    int grid = 0;
-    int64_t seq = 0;
-    int64_t batch_ns = 0;              // sum of launch -> completion seen, over batches
-    std::atomic<int64_t> gen{0};
-    std::atomic<int64_t> queued{0};
-    std::atomic<int64_t> finished{0};  // requests finished so far (done, failed, cancelled)
-    // Dispatcher time accounting (written by the dispatcher only).
-    std::atomic<int64_t> t_admit{0}, t_launch{0}, t_retire{0}, t_idle{0}, idle_launches{0};
-    std::atomic<bool> shutdown{false};
-    std::thread worker;
-    std::string fatal;
-
-    // Called under `mu` after any change to the queue or the arena.
-    void changed() {
-        queued.store((int64_t)core->queued(), std::memory_order_release);
-        gen.fetch_add(1, std::memory_order_acq_rel);
-    }
+    bs::HipExec exec;
+    bs::serve::Loop<bs::HipExec>* loop = nullptr;
 };
 
 using namespace bs;
@@ -206,186 +303,13 @@ using namespace bs;
 namespace {
 
 static_assert(serve::kErrBudget == BS_ERR_BUDGET && serve::kErrBusy == BS_ERR_BUSY &&
-                  serve::kErrNotFound == BS_ERR_NOT_FOUND &&
+                  serve::kErrNotFound == BS_ERR_NOT_FOUND && serve::kErrTimeout == BS_ERR_TIMEOUT &&
+                  serve::kErrCancelled == BS_ERR_CANCELLED && serve::kErrFailed == BS_ERR_FAILED &&
                   serve::kErrShutdown == BS_ERR_SHUTDOWN && serve::kDone == BS_TICKET_DONE &&
                   serve::kCancelled == BS_TICKET_CANCELLED && serve::kGreeks == BS_REQ_GREEKS,
               "serve_core codes mirror bs_mi355.h");
 
-// Launches one slot on stream st: graph replay, or the stage kernel and the
-// fused kernel of each non-empty section directly, each section's grid one
-// block per tile.
-hipError_t launch_slot(bs_ctx_s* c, int s, hipStream_t st) {
-    if (c->graph[s]) return hipGraphLaunch(c->graph[s], st);
-    const SlotHeader* h = reinterpret_cast<const SlotHeader*>(c->host_slot[s]);
-    hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, st,
-                       reinterpret_cast<const uint4*>(c->host_slot_dev[s]),
-                       reinterpret_cast<uint4*>(c->dev_slot[s]));
-    if (h->tiles[0] > 0)
-        hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3((unsigned)std::min(h->tiles[0], kMaxGrid)),
-                           dim3(kThreads), 0, st, c->dev_slot[s]);
-    if (h->tiles[1] > 0)
-        hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3((unsigned)std::min(h->tiles[1], kMaxGrid)),
-                           dim3(kThreads), 0, st, c->dev_slot[s]);
-    return hipGetLastError();
-}
-
-// Captures slot s as one graph: stage, price section, Greeks section, each
-// fused kernel over the fixed persistent grid c->grid.
-hipError_t capture_slot(bs_ctx_s* c, int s) {
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, c->stream,
-                       reinterpret_cast<const uint4*>(c->host_slot_dev[s]),
-                       reinterpret_cast<uint4*>(c->dev_slot[s]));
-    hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3(c->grid), dim3(kThreads), 0, c->stream,
-                       c->dev_slot[s]);
-    hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3(c->grid), dim3(kThreads), 0, c->stream,
-                       c->dev_slot[s]);
-    e = hipStreamEndCapture(c->stream, &g);
-    if (e != hipSuccess) return e;
-    e = hipGraphInstantiate(&c->graph[s], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    return e;
-}
-
-// Fills slot s with the batch's descriptors, price requests first, each
-// section in admission order.  Runs without the lock: it reads only the
-// admitted requests' copies, and the slot is the dispatcher's own.
-void write_slot(bs_ctx_s* c, int s, const std::vector<serve::Admitted>& reqs) {
-    SlotHeader* h = reinterpret_cast<SlotHeader*>(c->host_slot[s]);
-    Desc* d = reinterpret_cast<Desc*>(c->host_slot[s] + sizeof(SlotHeader));
-    int nd[2] = {0, 0};
-    for (const serve::Admitted& r : reqs) ++nd[r.kind == serve::kGreeks ? 1 : 0];
-    int next[2] = {0, nd[0]};
-    int64_t tiles[2] = {0, 0};
-    for (const serve::Admitted& r : reqs) {
-        const int sec = r.kind == serve::kGreeks ? 1 : 0;
-        size_t bo[6], oo[5];
-        serve::book_layout(r.book_n, bo);
-        serve::output_layout(r.n, r.kind, oo);
-        Desc& x = d[next[sec]++];
-        for (int j = 0; j < 5; ++j) x.in[j] = reinterpret_cast<const float*>(c->arena + r.book_off + bo[j]);
-        x.type = reinterpret_cast<const uint8_t*>(c->arena + r.book_off + bo[5]);
-        for (int j = 0; j < 5; ++j) x.out[j] = reinterpret_cast<float*>(c->arena + r.out_off + oo[j]);
-        x.n = r.n;
-        x.tile_begin = tiles[sec];
-        x.kind = r.kind;
-        x.pad = 0;
-        tiles[sec] += (r.n + kTile - 1) / kTile;
-    }
-    h->ndesc[0] = nd[0];
-    h->ndesc[1] = nd[1];
-    h->tiles[0] = tiles[0];
-    h->tiles[1] = tiles[1];
-    h->seq = ++c->seq;
-}
-
-void dispatcher(bs_ctx_s* c) {
-    (void)hipSetDevice(c->device);
-    std::vector<Batch> inflight;        // batches on the GPU; owned by this thread
-    bool busy[kSlots] = {};
-    int64_t blocked_gen = -1;           // gen at the last admission that found no room
-    int64_t t_empty = -1;               // when the last batch in flight retired
-    int64_t nlaunch = 0;                // batches alternate over the two streams
-    std::vector<serve::Admitted> reqs;  // reused: no per-batch allocation
-    reqs.reserve(kMaxDesc);
-    for (;;) {
-        bool progressed = false;
-        // retire every finished batch (batches on the two streams may finish
-        // out of launch order; each is independent of the others)
-        for (size_t bi = 0; bi < inflight.size();) {
-            Batch& b = inflight[bi];
-            const hipError_t q = hipEventQuery(b.done);
-            if (q == hipErrorNotReady) {
-                ++bi;
-                continue;
-            }
-            const int64_t t = now_ns();
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                if (q != hipSuccess) c->fatal = std::string("batch failed: ") + hipGetErrorString(q);
-                c->core->complete(b.tickets, q == hipSuccess, t);
-                c->batch_ns += t - b.t_launch;
-                c->changed();
-                c->finished.fetch_add((int64_t)b.tickets.size(), std::memory_order_release);
-            }
-            c->cv_done.notify_all();
-            busy[b.slot] = false;
-            inflight.erase(inflight.begin() + (std::ptrdiff_t)bi);
-            const int64_t t2 = now_ns();
-            c->t_retire.fetch_add(t2 - t, std::memory_order_relaxed);
-            if (inflight.empty()) t_empty = t2;
-            blocked_gen = -1;   // a failed batch frees its outputs
-            progressed = true;
-        }
-        // launch the next batch into a free slot
-        int s = -1;
-        for (int j = 0; j < kSlots && s < 0; ++j)
-            if (!busy[j]) s = j;
-        if (s >= 0 && !c->shutdown.load(std::memory_order_acquire) &&
-            c->queued.load(std::memory_order_acquire) > 0 &&
-            c->gen.load(std::memory_order_acquire) != blocked_gen) {
-            Batch b{s, c->ev[s], 0, {}};
-            const int64_t ta = now_ns();
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                const int64_t g = c->gen.load(std::memory_order_relaxed);
-                const int64_t qo = c->core->queued_options();
-                c->core->admit(now_ns(), &reqs, inflight.size() < 2 && qo > kSplitOptions ? qo / 2 : 0);
-                if (reqs.empty()) blocked_gen = g;   // the queue head does not fit: wait for a change
-                else c->changed();
-            }
-            if (!reqs.empty()) {
-                write_slot(c, s, reqs);
-                b.tickets.reserve(reqs.size());
-                for (const auto& r : reqs) b.tickets.push_back(r.ticket);
-                b.t_launch = now_ns();
-                c->t_admit.fetch_add(b.t_launch - ta, std::memory_order_relaxed);
-                if (inflight.empty() && t_empty >= 0) {
-                    c->t_idle.fetch_add(b.t_launch - t_empty, std::memory_order_relaxed);
-                    c->idle_launches.fetch_add(1, std::memory_order_relaxed);
-                }
-                hipStream_t st = c->streams[nlaunch++ & 1];
-                hipError_t e = launch_slot(c, s, st);
-                if (e == hipSuccess) e = hipEventRecord(c->ev[s], st);
-                c->t_launch.fetch_add(now_ns() - b.t_launch, std::memory_order_relaxed);
-                if (e != hipSuccess) {
-                    {
-                        std::lock_guard<std::mutex> lk(c->mu);
-                        c->fatal = std::string("launch failed: ") + hipGetErrorString(e);
-                        c->core->complete(b.tickets, false, now_ns());
-                        c->changed();
-                        c->finished.fetch_add((int64_t)b.tickets.size(), std::memory_order_release);
-                    }
-                    c->cv_done.notify_all();
-                } else {
-                    busy[s] = true;
-                    inflight.push_back(std::move(b));
-                }
-                progressed = true;
-            }
-        }
-        if (progressed) continue;
-        if (!inflight.empty()) {
-            std::this_thread::yield();   // a batch is on the GPU: keep polling its event
-            continue;
-        }
-        if (c->shutdown.load(std::memory_order_acquire)) break;
-        std::unique_lock<std::mutex> lk(c->mu);
-        c->cv_work.wait(lk, [&] {
-            return c->shutdown.load(std::memory_order_relaxed) ||
-                   (c->core->queued() > 0 && c->gen.load(std::memory_order_relaxed) != blocked_gen);
-        });
-    }
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        c->core->fail_queued(serve::kErrShutdown, now_ns());
-        c->changed();
-        c->finished.fetch_add(1, std::memory_order_release);
-    }
-    c->cv_done.notify_all();
-}
+int status(int rc, const char* what) { return rc == BS_OK ? BS_OK : set_error(rc, what); }
 
 }  // namespace
 
@@ -400,50 +324,28 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || device < 0 || device >= ndev)
         return set_error(BS_ERR_NODEVICE, "device index out of range");
-    auto* c = new bs_ctx_s();
-    c->device = device;
-    c->opts.max_batch_requests = kMaxDesc;
-    c->opts.max_batch_options = int64_t(1) << 31;
-    c->opts.use_graph = 0;
-    c->opts.grid_blocks = 0;
+    serve::Limits lim;
+    lim.max_batch_requests = kMaxDesc;
+    lim.max_batch_options = int64_t(1) << 31;
+    int use_graph = 0, grid_blocks = 0;
     if (opts) {
         if (opts->max_batch_requests > 0)
-            c->opts.max_batch_requests = std::min<int64_t>(opts->max_batch_requests, kMaxDesc);
-        if (opts->max_batch_options > 0) c->opts.max_batch_options = opts->max_batch_options;
-        c->opts.use_graph = opts->use_graph;
-        c->opts.grid_blocks = opts->grid_blocks;
+            lim.max_batch_requests = std::min<int64_t>(opts->max_batch_requests, kMaxDesc);
+        if (opts->max_batch_options > 0) lim.max_batch_options = opts->max_batch_options;
+        use_graph = opts->use_graph;
+        grid_blocks = opts->grid_blocks;
     }
-    int rc = BS_OK;
-    e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipMalloc((void**)&c->arena, hbm_budget_bytes);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int j = 0; j < 2 && e == hipSuccess; ++j)
-        e = hipStreamCreateWithFlags(&c->streams[j], hipStreamNonBlocking);
-    for (int s = 0; s < kSlots && e == hipSuccess; ++s) {
-        e = hipHostMalloc((void**)&c->host_slot[s], kSlotBytes, hipHostMallocCoherent);
-        if (e == hipSuccess)
-            e = hipHostGetDevicePointer((void**)&c->host_slot_dev[s], c->host_slot[s], 0);
-        if (e == hipSuccess) e = hipMalloc((void**)&c->dev_slot[s], kSlotBytes);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[s], hipEventDisableTiming);
-        if (e == hipSuccess) ::memset(c->host_slot[s], 0, sizeof(SlotHeader));
-    }
-    if (e == hipSuccess) {
-        int cus = 0;
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-        c->grid = c->opts.grid_blocks > 0 ? c->opts.grid_blocks : std::max(1, cus) * 8;
-    }
-    for (int s = 0; s < kSlots && e == hipSuccess && c->opts.use_graph; ++s) e = capture_slot(c, s);
+    auto* c = new bs_ctx_s();
+    e = c->exec.init(device, hbm_budget_bytes, use_graph, grid_blocks);
     if (e != hipSuccess) {
-        rc = hip_error(e, "bs_ctx_create");
-        bs_ctx_destroy(c);
+        const int rc = hip_error(e, "bs_ctx_create");
+        c->exec.release();
+        delete c;
         return rc;
     }
-    serve::Limits lim;
-    lim.max_batch_requests = c->opts.max_batch_requests;
-    lim.max_batch_options = c->opts.max_batch_options;
-    c->core = new serve::Core(hbm_budget_bytes, lim);
-    c->core->enable_completions(opts && opts->completion_queue != 0);
-    c->worker = std::thread(dispatcher, c);
+    c->loop = new serve::Loop<HipExec>(&c->exec, hbm_budget_bytes, lim,
+                                       opts && opts->completion_queue != 0);
+    c->loop->start();
     *out = c;
     return BS_OK;
 }
@@ -451,30 +353,9 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
 int bs_ctx_destroy(bs_ctx_t ctx) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    if (c->worker.joinable()) {
-        {
-            std::lock_guard<std::mutex> lk(c->mu);
-            c->shutdown.store(true);
-            c->gen.fetch_add(1);
-        }
-        c->cv_work.notify_all();
-        c->worker.join();
-    }
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (hipStream_t st : c->streams)
-        if (st) (void)hipStreamSynchronize(st);
-    for (int s = 0; s < kSlots; ++s) {
-        if (c->graph[s]) (void)hipGraphExecDestroy(c->graph[s]);
-        if (c->ev[s]) (void)hipEventDestroy(c->ev[s]);
-        if (c->host_slot[s]) (void)hipHostFree(c->host_slot[s]);
-        if (c->dev_slot[s]) (void)hipFree(c->dev_slot[s]);
-    }
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    for (hipStream_t st : c->streams)
-        if (st) (void)hipStreamDestroy(st);
-    if (c->arena) (void)hipFree(c->arena);
-    delete c->core;
+    if (c->loop) c->loop->stop();
+    c->exec.release();
+    delete c->loop;
     delete c;
     return BS_OK;
 }
@@ -487,23 +368,18 @@ int bs_book_register(bs_ctx_t ctx, const float* s, const float* k, const float* 
     if (!s || !k || !r || !v || !t || !otype) return set_error(BS_ERR_INVALID, "NULL array");
     int64_t id;
     size_t off;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
-        const int rc = c->core->register_book(n, &id, &off);
-        if (rc != BS_OK) return set_error(rc, "book does not fit in the arena");
-    }
+    const int rc = c->loop->register_book(n, &id, &off);
+    if (rc == BS_ERR_SHUTDOWN) return set_error(rc, "context shutting down");
+    if (rc != BS_OK) return set_error(rc, "book does not fit in the arena");
     size_t bo[6];
     serve::book_layout(n, bo);
     const void* src[6] = {s, k, r, v, t, otype};
-    hipError_t e = hipSetDevice(c->device);
+    hipError_t e = hipSetDevice(c->exec.device);
     for (int j = 0; j < 6 && e == hipSuccess; ++j)
-        e = hipMemcpy(c->arena + off + bo[j], src[j], (size_t)n * (j < 5 ? 4 : 1),
+        e = hipMemcpy(c->exec.arena + off + bo[j], src[j], (size_t)n * (j < 5 ? 4 : 1),
                       hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        std::lock_guard<std::mutex> lk(c->mu);
-        c->core->release_book(id);
-        c->changed();
+        c->loop->release_book(id);
         return hip_error(e, "bs_book_register");
     }
     *book = id;
@@ -513,29 +389,13 @@ int bs_book_register(bs_ctx_t ctx, const float* s, const float* k, const float* 
 int bs_book_release(bs_ctx_t ctx, bs_book_t book) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    int rc;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        rc = c->core->release_book(book);
-        if (rc == BS_OK) c->changed();
-    }
-    if (rc == BS_OK) c->cv_work.notify_all();
-    return rc == BS_OK ? BS_OK : set_error(rc, "bs_book_release");
+    return status(c->loop->release_book(book), "bs_book_release");
 }
 
 int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t* ticket) {
     bs_ctx_s* c = ctx;
     if (!c || !ticket) return set_error(BS_ERR_INVALID, "ctx or ticket is NULL");
-    int rc;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
-        rc = c->core->submit(book, kind, now_ns(), ticket);
-        if (rc == BS_OK) c->changed();
-    }
-    if (rc != BS_OK) return set_error(rc, "bs_submit");
-    c->cv_work.notify_one();
-    return BS_OK;
+    return status(c->loop->submit(book, kind, ticket), "bs_submit");
 }
 
 int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, const int64_t* users,
@@ -544,17 +404,7 @@ int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, const
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
     if (count < 0 || (count > 0 && (!books || !kinds || !tickets)))
         return set_error(BS_ERR_INVALID, "bad request arrays");
-    if (count == 0) return BS_OK;
-    int rc;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        if (c->shutdown.load()) return set_error(BS_ERR_SHUTDOWN, "context shutting down");
-        rc = c->core->submit_many(books, kinds, users, count, now_ns(), tickets);
-        if (rc == BS_OK) c->changed();
-    }
-    if (rc != BS_OK) return set_error(rc, "bs_submit_many");
-    c->cv_work.notify_one();
-    return BS_OK;
+    return status(c->loop->submit_many(books, kinds, users, count, tickets), "bs_submit_many");
 }
 
 int bs_reap(bs_ctx_t ctx, bs_completion* out, int64_t max, int64_t timeout_us, int64_t* count) {
@@ -567,106 +417,54 @@ int bs_reap(bs_ctx_t ctx, bs_completion* out, int64_t max, int64_t timeout_us, i
     if (!c || !count) return set_error(BS_ERR_INVALID, "ctx or count is NULL");
     *count = 0;
     if (max < 0 || (max > 0 && !out)) return set_error(BS_ERR_INVALID, "bad output array");
-    auto* dst = reinterpret_cast<serve::Completion*>(out);
-    const auto t0 = std::chrono::steady_clock::now();
-    const auto deadline = t0 + std::chrono::microseconds(timeout_us);
-    std::unique_lock<std::mutex> lk(c->mu);
-    if (!c->core->completions_enabled())
-        return set_error(BS_ERR_INVALID, "context has no completion queue");
-    if (max == 0) return BS_OK;
-    for (;;) {
-        const int64_t seen = c->finished.load(std::memory_order_acquire);
-        const int64_t k = c->core->reap(dst, max);
-        if (k > 0 || timeout_us == 0) {
-            *count = k;
-            return BS_OK;
-        }
-        // Spin for up to kReapSpinUs on the finished counter without the lock
-        // (a batch completes every few hundred microseconds under load, and a
-        // sleeping reaper costs a scheduler wake-up per batch), then sleep.
-        lk.unlock();
-        const auto now = std::chrono::steady_clock::now();
-        const auto spin_end = std::min(now + std::chrono::microseconds(kReapSpinUs),
-                                       timeout_us < 0 ? now + std::chrono::hours(1) : deadline);
-        while (c->finished.load(std::memory_order_acquire) == seen &&
-               std::chrono::steady_clock::now() < spin_end)
-            std::this_thread::yield();
-        lk.lock();
-        if (c->finished.load(std::memory_order_acquire) != seen ||
-            c->core->completions_pending() > 0)
-            continue;
-        auto ready = [&] { return c->core->completions_pending() > 0; };
-        if (timeout_us < 0) {
-            c->cv_done.wait(lk, ready);
-        } else if (!c->cv_done.wait_until(lk, deadline, ready)) {
-            return BS_OK;
-        }
-    }
+    const int rc = c->loop->reap(reinterpret_cast<serve::Completion*>(out), max, timeout_us, count);
+    return rc == BS_OK ? BS_OK : set_error(rc, "context has no completion queue");
 }
 
 int bs_result_release_many(bs_ctx_t ctx, const bs_ticket_t* tickets, int64_t count) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
     if (count < 0 || (count > 0 && !tickets)) return set_error(BS_ERR_INVALID, "bad ticket array");
-    int first = BS_OK;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        for (int64_t i = 0; i < count; ++i) {
-            const int rc = c->core->release_result(tickets[i]);
-            if (rc != BS_OK && first == BS_OK) first = rc;
-        }
-        c->changed();
-    }
-    c->cv_work.notify_all();
-    return first == BS_OK ? BS_OK : set_error(first, "bs_result_release_many");
+    return status(c->loop->release_many(tickets, count), "bs_result_release_many");
 }
 
 int bs_poll(bs_ctx_t ctx, bs_ticket_t ticket, int* state) {
     bs_ctx_s* c = ctx;
     if (!c || !state) return set_error(BS_ERR_INVALID, "ctx or state is NULL");
-    std::lock_guard<std::mutex> lk(c->mu);
-    const serve::Request* r = c->core->request(ticket);
-    if (!r) return set_error(BS_ERR_NOT_FOUND, "unknown ticket");
-    *state = r->state;
-    return BS_OK;
+    return status(c->loop->poll(ticket, state), "unknown ticket");
 }
 
 int bs_wait(bs_ctx_t ctx, bs_ticket_t ticket, int64_t timeout_us) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    std::unique_lock<std::mutex> lk(c->mu);
-    auto finished = [&]() {
-        const serve::Request* r = c->core->request(ticket);
-        return !r || (r->state != serve::kQueued && r->state != serve::kRunning);
-    };
-    if (timeout_us < 0) {
-        c->cv_done.wait(lk, finished);
-    } else if (!c->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), finished)) {
-        return set_error(BS_ERR_TIMEOUT, "bs_wait timed out");
+    std::string msg;
+    const int rc = c->loop->wait(ticket, timeout_us, &msg);
+    switch (rc) {
+        case BS_OK: return BS_OK;
+        case BS_ERR_TIMEOUT: return set_error(rc, "bs_wait timed out");
+        case BS_ERR_NOT_FOUND: return set_error(rc, "unknown ticket");
+        case BS_ERR_CANCELLED: return set_error(rc, "ticket cancelled");
+        default: return set_error(rc, msg);
     }
-    const serve::Request* r = c->core->request(ticket);
-    if (!r) return set_error(BS_ERR_NOT_FOUND, "unknown ticket");
-    if (r->state == serve::kDone) return BS_OK;
-    if (r->state == serve::kCancelled) return set_error(BS_ERR_CANCELLED, "ticket cancelled");
-    return set_error(r->status == BS_OK ? BS_ERR_FAILED : r->status,
-                     c->fatal.empty() ? "request failed" : c->fatal);
 }
 
 int bs_result_device(bs_ctx_t ctx, bs_ticket_t ticket, const float** price, const float** delta,
                      const float** vega, const float** rho, const float** theta) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    std::lock_guard<std::mutex> lk(c->mu);
-    const serve::Request* r = c->core->request(ticket);
-    if (!r) return set_error(BS_ERR_NOT_FOUND, "unknown ticket");
-    if (r->state != serve::kDone) return set_error(BS_ERR_INVALID, "ticket not done");
+    int64_t n;
+    int kind;
+    size_t off;
+    const int rc = c->loop->result_info(ticket, &n, &kind, &off);
+    if (rc == BS_ERR_NOT_FOUND) return set_error(rc, "unknown ticket");
+    if (rc != BS_OK) return set_error(rc, "ticket not done");
     size_t oo[5];
-    serve::output_layout(r->n, r->kind, oo);
+    serve::output_layout(n, kind, oo);
     const float** dst[5] = {price, delta, vega, rho, theta};
     for (int j = 0; j < 5; ++j) {
         if (!dst[j]) continue;
-        *dst[j] = (j == 0 || r->kind == serve::kGreeks)
-                      ? reinterpret_cast<const float*>(c->arena + r->out_off + oo[j])
+        *dst[j] = (j == 0 || kind == serve::kGreeks)
+                      ? reinterpret_cast<const float*>(c->exec.arena + off + oo[j])
                       : nullptr;
     }
     return BS_OK;
@@ -674,27 +472,22 @@ int bs_result_device(bs_ctx_t ctx, bs_ticket_t ticket, const float** price, cons
 
 int bs_result_copy(bs_ctx_t ctx, bs_ticket_t ticket, float* price, float* delta, float* vega,
                    float* rho, float* theta) {
-    const float* src[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    int64_t n = 0;
-    int kind = 0;
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        const serve::Request* r = c->core->request(ticket);
-        if (!r) return set_error(BS_ERR_NOT_FOUND, "unknown ticket");
-        if (r->state != serve::kDone) return set_error(BS_ERR_INVALID, "ticket not done");
-        n = r->n;
-        kind = r->kind;
-    }
-    int rc = bs_result_device(ctx, ticket, &src[0], &src[1], &src[2], &src[3], &src[4]);
-    if (rc != BS_OK) return rc;
+    int64_t n;
+    int kind;
+    size_t off;
+    const int rc = c->loop->result_info(ticket, &n, &kind, &off);
+    if (rc == BS_ERR_NOT_FOUND) return set_error(rc, "unknown ticket");
+    if (rc != BS_OK) return set_error(rc, "ticket not done");
+    size_t oo[5];
+    serve::output_layout(n, kind, oo);
     float* dst[5] = {price, delta, vega, rho, theta};
-    hipError_t e = hipSetDevice(c->device);
+    hipError_t e = hipSetDevice(c->exec.device);
     for (int j = 0; j < 5 && e == hipSuccess; ++j) {
         if (!dst[j]) continue;
         if (j > 0 && kind != serve::kGreeks) return set_error(BS_ERR_INVALID, "not a Greeks ticket");
-        e = hipMemcpy(dst[j], src[j], (size_t)n * 4, hipMemcpyDeviceToHost);
+        e = hipMemcpy(dst[j], c->exec.arena + off + oo[j], (size_t)n * 4, hipMemcpyDeviceToHost);
     }
     return e == hipSuccess ? BS_OK : hip_error(e, "bs_result_copy");
 }
@@ -702,54 +495,31 @@ int bs_result_copy(bs_ctx_t ctx, bs_ticket_t ticket, float* price, float* delta,
 int bs_result_release(bs_ctx_t ctx, bs_ticket_t ticket) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    int rc;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        rc = c->core->release_result(ticket);
-        if (rc == BS_OK) c->changed();
-    }
-    if (rc == BS_OK) c->cv_work.notify_all();
-    return rc == BS_OK ? BS_OK : set_error(rc, "bs_result_release");
+    return status(c->loop->release(ticket), "bs_result_release");
 }
 
 int bs_cancel(bs_ctx_t ctx, bs_ticket_t ticket) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    int rc;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        rc = c->core->cancel(ticket, now_ns());
-        if (rc == BS_OK) {
-            c->changed();
-            c->finished.fetch_add(1, std::memory_order_release);
-        }
-    }
-    if (rc == BS_OK) {
-        c->cv_done.notify_all();
-        c->cv_work.notify_all();   // the cancelled request may have blocked the queue head
-    }
-    return rc == BS_OK ? BS_OK : set_error(rc, "bs_cancel: ticket not queued");
+    return status(c->loop->cancel(ticket), "bs_cancel: ticket not queued");
 }
 
 int bs_ticket_times(bs_ctx_t ctx, bs_ticket_t ticket, int64_t* submit_ns, int64_t* done_ns) {
     bs_ctx_s* c = ctx;
     if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    std::lock_guard<std::mutex> lk(c->mu);
-    const serve::Request* r = c->core->request(ticket);
-    if (!r) return set_error(BS_ERR_NOT_FOUND, "unknown ticket");
-    if (submit_ns) *submit_ns = r->t_submit;
-    if (done_ns) *done_ns = r->t_done;
-    return BS_OK;
+    return status(c->loop->times(ticket, submit_ns, done_ns), "unknown ticket");
 }
 
 int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats* out) {
     bs_ctx_s* c = ctx;
     if (!c || !out) return set_error(BS_ERR_INVALID, "ctx or stats is NULL");
-    std::lock_guard<std::mutex> lk(c->mu);
-    const serve::Stats st = c->core->stats();
-    out->arena_bytes = (int64_t)c->core->arena().capacity();
-    out->arena_used = (int64_t)c->core->arena().used();
-    out->arena_peak = (int64_t)c->core->arena().peak();
+    serve::Stats st;
+    serve::LoopTimes lt;
+    size_t cap, used, peak;
+    c->loop->stats(&st, &cap, &used, &peak, &lt);
+    out->arena_bytes = (int64_t)cap;
+    out->arena_used = (int64_t)used;
+    out->arena_peak = (int64_t)peak;
     out->books_resident = st.books_resident;
     out->requests_queued = st.requests_queued;
     out->requests_running = st.requests_running;
@@ -760,12 +530,12 @@ int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats* out) {
     out->options_priced = st.options_priced;
     out->admission_waits = st.admission_waits;
     out->queue_ns_total = st.queue_ns;
-    out->batch_ns_total = c->batch_ns;
-    out->dispatch_admit_ns = c->t_admit.load();
-    out->dispatch_launch_ns = c->t_launch.load();
-    out->dispatch_retire_ns = c->t_retire.load();
-    out->gpu_idle_ns = c->t_idle.load();
-    out->idle_launches = c->idle_launches.load();
+    out->batch_ns_total = lt.batch;
+    out->dispatch_admit_ns = lt.admit;
+    out->dispatch_launch_ns = lt.launch;
+    out->dispatch_retire_ns = lt.retire;
+    out->gpu_idle_ns = lt.idle;
+    out->idle_launches = lt.idle_launches;
     return BS_OK;
 }
 

@@ -72,6 +72,7 @@ enum : int {
     kErrBudget = -4,
     kErrNotFound = -5,
     kErrCancelled = -6,
+    kErrTimeout = -7,
     kErrBusy = -8,
     kErrFailed = -10,
     kErrShutdown = -11
