@@ -28,7 +28,8 @@
 //            v_rcp_f32, v_sqrt_f32), Horner polynomial with FMA.  About a
 //            third of Ref32's VALU work; prices differ from the reference by
 //            up to ~1.2e-4 on a 10M book (mean 6e-6).
-//   Exact64  fptype = double: every step in FP64, no FMA contraction.
+//   Ref64    fptype = double: the same statements in FP64, no FMA contraction
+//            (its log and CNDF reciprocal are written out below).
 //
 // Every function here is compiled with FMA contraction off; each FMA is
 // written out, so every kernel that inlines these functions computes the same
@@ -203,31 +204,96 @@ struct Fast32 {
 };
 
 // ---------------------------------------------------------------------------
-// Exact64: fptype = double.
-struct Exact64 {
+// FP64 primitives for the fptype = double policy.
+
+// 1/b for finite b with |b| in [2^-1000, 2^1000]: the hardware reciprocal
+// (v_rcp_f64) refined by two Newton steps y <- y + y (1 - b y), each residual
+// in one FMA.  NaN propagates.
+__device__ __forceinline__ double recip64(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+constexpr double kSqrtHalf = 0.70710678118654752440;
+// ln 2 = kLn2Hi + kLn2Lo; kLn2Hi keeps 32 significant bits, so e * kLn2Hi is
+// exact for every binary exponent e of a double.
+constexpr double kLn2Hi = 0.6931471803691238;        // 0x3fe62e42fee00000
+constexpr double kLn2Lo = 1.9082149292705877e-10;
+
+// Natural logarithm.  x = 2^e m with m in [sqrt(1/2), sqrt(2)); f = m - 1 is
+// exact and s = f / (2 + f), |s| <= 0.1716.  Then
+//   log m = 2 atanh(s) = f - h + s (h + R),   h = f^2 / 2,
+//   R = sum_{j=1..10} 2 s^(2j) / (2j + 1)
+// (the Taylor series of atanh; the first omitted term is below 6.3e-19 of
+// log m), and log x = e kLn2Hi + (log m + e kLn2Lo).  s is the quotient from
+// recip64 with one residual correction.  Zero gives -inf, +inf gives +inf,
+// negative arguments and NaN give NaN, as IEEE log.
+__device__ __forceinline__ double log64(double x) {
+#pragma clang fp contract(off)
+    double m = __builtin_amdgcn_frexp_mant(x);    // [0.5, 1), subnormals too
+    int e = __builtin_amdgcn_frexp_exp(x);
+    const bool lo = m < kSqrtHalf;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double f = m - 1.0;
+    const double d = f + 2.0;
+    const double y = recip64(d);
+    double s = f * y;
+    s = __builtin_fma(__builtin_fma(-d, s, f), y, s);
+    const double z = s * s;
+    double p = 2.0 / 21.0;
+    p = __builtin_fma(p, z, 2.0 / 19.0);
+    p = __builtin_fma(p, z, 2.0 / 17.0);
+    p = __builtin_fma(p, z, 2.0 / 15.0);
+    p = __builtin_fma(p, z, 2.0 / 13.0);
+    p = __builtin_fma(p, z, 2.0 / 11.0);
+    p = __builtin_fma(p, z, 2.0 / 9.0);
+    p = __builtin_fma(p, z, 2.0 / 7.0);
+    p = __builtin_fma(p, z, 2.0 / 5.0);
+    p = __builtin_fma(p, z, 2.0 / 3.0);
+    const double rr = z * p;
+    const double h = 0.5 * (f * f);
+    const double lm = f + __builtin_fma(s, h + rr, -h);
+    const double ed = (double)e;
+    const double r = __builtin_fma(ed, kLn2Hi, __builtin_fma(ed, kLn2Lo, lm));
+    if (x > 0.0 && x < __builtin_inf()) return r;
+    return x == 0.0 ? -__builtin_inf() : (x > 0.0 ? x : __builtin_nan(""));
+}
+
+// Ref64: fptype = double, the reference's statement order with every step in
+// FP64 and no FMA contraction.  Division and square root are IEEE (the
+// compiler's correctly rounded sequences) and exp is the device library's;
+// log is log64 and CNDF's reciprocal 1 / (1 + p|x|) is recip64, its argument
+// capped at 1e10 (past |x| = 38.6 the result is already exactly 0 or 1, and
+// the cap keeps the denominator finite).  Against the reference compiled with
+// fptype = double, prices differ by a few units in the last place.
+struct Ref64 {
     using T = double;
     __device__ static __forceinline__ double exp(double x) { return ::exp(x); }
-    __device__ static __forceinline__ double log(double x) { return ::log(x); }
+    __device__ static __forceinline__ double log(double x) { return log64(x); }
     __device__ static __forceinline__ double div(double a, double b) { return a / b; }
     __device__ static __forceinline__ double sqrt(double x) { return ::sqrt(x); }
 
     template <bool kGrad>
     __device__ static __forceinline__ double cndf(double x, double* grad) {
 #pragma clang fp contract(off)
-        static_assert(!kGrad, "Exact64 prices only");
+        static_assert(!kGrad, "Ref64 prices only");
         const bool neg = x < 0.0;
-        if (neg) x = -x;
-        double n = exp(-0.5 * x * x) * kInvSqrt2Pi;
-        double k = 1.0 / (1.0 + kP * x);
+        double ax = neg ? -x : x;
+        ax = ax > 1e10 ? 1e10 : ax;
+        const double n = exp(-0.5 * ax * ax) * kInvSqrt2Pi;
+        const double k = recip64(1.0 + kP * ax);
         const double k2 = k * k, k3 = k2 * k, k4 = k3 * k, k5 = k4 * k;
         double acc = k2 * kB2;
         acc = acc + k3 * kB3;
         acc = acc + k4 * kB4;
         acc = acc + k5 * kB5;
         const double t1 = acc + k * kB1;
-        double y = 1.0 - t1 * n;
-        if (neg) y = 1.0 - y;
-        return y;
+        const double y = 1.0 - t1 * n;
+        return neg ? 1.0 - y : y;
     }
 };
 

@@ -155,11 +155,11 @@ struct PriceF64 {
         D p;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-            p[j] = bs_price<Exact64>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
+            p[j] = bs_price<Ref64>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
         st<NT>(reinterpret_cast<D*>(out) + g, p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = bs_price<Exact64>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
+        out[i] = bs_price<Ref64>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
     }
 };
 
@@ -268,12 +268,12 @@ struct CndfF64 {
     template <bool NT>
     __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
         D y;
-        y[0] = Exact64::cndf<false>(a.x[0], nullptr);
-        y[1] = Exact64::cndf<false>(a.x[1], nullptr);
+        y[0] = Ref64::cndf<false>(a.x[0], nullptr);
+        y[1] = Ref64::cndf<false>(a.x[1], nullptr);
         st<NT>(reinterpret_cast<D*>(out) + g, y);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
-        out[i] = Exact64::cndf<false>(x[i], nullptr);
+        out[i] = Ref64::cndf<false>(x[i], nullptr);
     }
 };
 

@@ -5,6 +5,7 @@ reference's ERR_CHK threshold, blackscholes.c:333-340); the fptype=double map
 within 1e-9 of the FP64 restatement; the launch shape (alignment, grid cap,
 unroll, non-temporal) never changes a bit.  Every test calls through the C ABI.
 """
+import os
 import ctypes
 
 import numpy as np
@@ -262,3 +263,16 @@ def test_invalid_pointers_rejected_on_device(gpu):
     assert L.bs_price_f32(*p[:6], host.ctypes.data, 1000, None) == -1
     gpu.call("bs_price_f32", *p, 1000, None)
     gpu.call("bs_stream_sync", None)
+
+
+def test_fp64_primitives_against_the_c_library(gpu):
+    """The FP64 policy's log64 and recip64 on the GPU against the host C
+    library over ~8.7M log and ~8.4M reciprocal inputs (every binade,
+    subnormals, [0.5, 2], 1 +- k ulp, special values): log64 within 2 ulp,
+    recip64 within 1 ulp, special values identical (tests/native/fp64_prims.hip)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "fp64_prims")
+    assert os.path.exists(exe), "build it: make -C tests/native fp64_prims"
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print("\n" + p.stdout.strip())
+    assert p.returncode == 0, p.stdout + p.stderr
