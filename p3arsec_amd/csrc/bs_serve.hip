@@ -3,7 +3,7 @@
 //
 // A context owns
 //   * one device arena (hipMalloc of the HBM budget) holding resident books
-//     and request outputs, carved by serve::Core's first-fit allocator;
+//     and request outputs, carved by serve::Core's segregated-fit allocator;
 //   * four descriptor slots, each a pinned host table plus its device copy;
 //   * per batch, a stage kernel that copies the slot's table from pinned host
 //     memory to the device, then one fused kernel per section (the batch's
@@ -13,10 +13,11 @@
 //     tile; with use_graph the three kernels are captured once per slot over
 //     a fixed persistent grid and each batch is one hipGraphLaunch (on ROCm
 //     7.2 the direct form measured 15 us launch-to-completion against 31 us);
-//   * a dispatcher thread: it admits queued requests as arena memory allows,
-//     writes a table into a free slot and launches it behind the batch still
-//     running (up to four batches in flight), and retires a batch when its event
-//     completes.  Requests that arrive while a batch runs go into the next one.
+//   * a dispatcher thread (serve_loop.hpp): it admits queued requests as arena
+//     memory allows, writes a table into a free slot and launches it, batches
+//     alternating over two streams (up to four in flight), and retires each
+//     batch when its event completes.  Requests that arrive while batches run
+//     go into the next one.
 // Results are bit-identical to bs_price_f32 / bs_price_greeks_f32: the fused
 // kernels run the batch map's own per-group code (bs_ops.hpp).
 #include <stddef.h>
@@ -378,6 +379,9 @@ int bs_book_register(bs_ctx_t ctx, const float* s, const float* k, const float* 
     for (int j = 0; j < 6 && e == hipSuccess; ++j)
         e = hipMemcpy(c->exec.arena + off + bo[j], src[j], (size_t)n * (j < 5 ? 4 : 1),
                       hipMemcpyHostToDevice);
+    // The batches run on non-blocking streams: the book must be in device
+    // memory before its id is handed out.
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         c->loop->release_book(id);
         return hip_error(e, "bs_book_register");
