@@ -202,7 +202,9 @@ typedef struct {
     int64_t max_batch_options;  /* options fused per launch                  */
     int32_t use_graph;          /* 1: replay a captured hipGraph per batch;
                                    0: direct launches (default, faster)     */
-    int32_t grid_blocks;        /* persistent grid size, 0 = auto            */
+    int32_t grid_blocks;        /* cap on each section's grid (one block per
+                                   512-option tile; blocks loop past the
+                                   cap), 0 = 2^20                            */
     int32_t completion_queue;   /* 1: finished requests queue for bs_reap    */
     int32_t reserved;           /* 0                                         */
 } bs_serve_opts;

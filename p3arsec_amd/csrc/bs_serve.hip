@@ -10,9 +10,10 @@
 //     price requests, its Greeks requests): a block prices one tile of 512
 //     options of one request, found by a 64-ary wave search of the table.
 //     Launched directly (the default) each section's grid is one block per
-//     tile; with use_graph the three kernels are captured once per slot over
-//     a fixed persistent grid and each batch is one hipGraphLaunch (on ROCm
-//     7.2 the direct form measured 15 us launch-to-completion against 31 us);
+//     tile.  With use_graph the three kernels are captured once per slot and
+//     each batch is one hipGraphLaunch; before it, each section's node gets
+//     the batch's grid (hipGraphExecKernelNodeSetParams).  On ROCm 7.2 the
+//     direct form measured 15 us launch-to-completion against 31 us;
 //   * a dispatcher thread (serve_loop.hpp): it admits queued requests as arena
 //     memory allows, writes a table into a free slot and launches it, batches
 //     alternating over two streams (up to four in flight), and retires each
@@ -40,7 +41,7 @@ namespace {
 constexpr int kVW = 2;                 // options per work-item (the batch map's width)
 constexpr int kTile = kThreads * kVW;  // options per tile: one block step
 constexpr int kMaxDesc = 4096;         // requests per batch: 64 x 64 for find_desc
-constexpr int64_t kMaxGrid = int64_t(1) << 20;  // direct-launch grid cap (blocks loop)
+constexpr int64_t kMaxGrid = int64_t(1) << 20;  // default section grid cap (blocks loop)
 constexpr int kStageBlocks = 32;
 
 // One request of a batch: 112 bytes, a multiple of the stage kernel's 16-byte words.
@@ -104,8 +105,9 @@ __device__ __forceinline__ GreeksF32<kVW> op_of(const Desc& x) {
 
 // Fused kernel of one section: a block prices tile t (kTile options of one
 // request, one kVW-group per work-item) for t = blockIdx.x, + gridDim.x, ...
-// Launched directly, the grid has one block per tile; replayed from a graph it
-// is a fixed persistent grid.  Every access is bounded by the request's n.
+// The grid has one block per tile up to the context's cap (kMaxGrid unless
+// bs_serve_opts.grid_blocks sets it), in both launch forms.
+// Every access is bounded by the request's n.
 // The per-group code is the batch map's own (bs_ops.hpp).
 template <class Op, int kSection>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, false>())))
@@ -145,7 +147,7 @@ struct HipExec {
 
     int device = 0;
     int use_graph = 0;
-    int grid = 0;
+    int64_t grid_cap = kMaxGrid;                // blocks per section at most
     char* arena = nullptr;
     hipStream_t stream = nullptr;               // capture stream
     hipStream_t streams[2] = {nullptr, nullptr};  // batches alternate over these
@@ -153,6 +155,10 @@ struct HipExec {
     uint8_t* host_slot_dev[kSlots] = {};        // the same memory, device address
     uint8_t* dev_slot[kSlots] = {};
     hipGraphExec_t graph[kSlots] = {};
+    hipGraph_t graph_src[kSlots] = {};          // kept: its nodes name the exec's nodes
+    hipGraphNode_t section_node[kSlots][2] = {};
+    unsigned section_grid[kSlots][2] = {};      // grid the exec's section node has now
+    void* section_args[kSlots][1] = {};         // kernel arguments: &dev_slot[s]
     hipEvent_t ev[kSlots] = {};
     int64_t seq = 0, nlaunch = 0;
 
@@ -172,11 +178,7 @@ struct HipExec {
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[s], hipEventDisableTiming);
             if (e == hipSuccess) ::memset(host_slot[s], 0, sizeof(SlotHeader));
         }
-        if (e == hipSuccess) {
-            int cus = 0;
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-            grid = grid_blocks > 0 ? grid_blocks : std::max(1, cus) * 8;
-        }
+        if (grid_blocks > 0) grid_cap = std::min<int64_t>(grid_blocks, kMaxGrid);
         for (int s = 0; s < kSlots && e == hipSuccess && use_graph; ++s) e = capture(s);
         return e;
     }
@@ -185,6 +187,7 @@ struct HipExec {
         drain();
         for (int s = 0; s < kSlots; ++s) {
             if (graph[s]) (void)hipGraphExecDestroy(graph[s]);
+            if (graph_src[s]) (void)hipGraphDestroy(graph_src[s]);
             if (ev[s]) (void)hipEventDestroy(ev[s]);
             if (host_slot[s]) (void)hipHostFree(host_slot[s]);
             if (dev_slot[s]) (void)hipFree(dev_slot[s]);
@@ -195,8 +198,8 @@ struct HipExec {
         if (arena) (void)hipFree(arena);
     }
 
-    // Captures slot s as one graph: stage, price section, Greeks section,
-    // each fused kernel over the fixed persistent grid.
+    // Captures slot s as one graph: stage, price section, Greeks section.  The
+    // section nodes are found by kernel so each launch can set their grids.
     hipError_t capture(int s) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
@@ -204,14 +207,45 @@ struct HipExec {
         hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, stream,
                            reinterpret_cast<const uint4*>(host_slot_dev[s]),
                            reinterpret_cast<uint4*>(dev_slot[s]));
-        hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3(grid), dim3(kThreads), 0, stream,
+        hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3(1), dim3(kThreads), 0, stream,
                            dev_slot[s]);
-        hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3(grid), dim3(kThreads), 0, stream,
+        hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3(1), dim3(kThreads), 0, stream,
                            dev_slot[s]);
         e = hipStreamEndCapture(stream, &g);
         if (e != hipSuccess) return e;
-        e = hipGraphInstantiate(&graph[s], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        graph_src[s] = g;
+        section_args[s][0] = &dev_slot[s];
+        const void* fn[2] = {reinterpret_cast<const void*>(&k_serve<ServePrice, 0>),
+                             reinterpret_cast<const void*>(&k_serve<ServeGreeks, 1>)};
+        hipGraphNode_t nodes[3];
+        size_t count = 3;
+        e = hipGraphGetNodes(g, nodes, &count);
+        for (size_t i = 0; i < count && e == hipSuccess; ++i) {
+            hipKernelNodeParams p{};
+            e = hipGraphKernelNodeGetParams(nodes[i], &p);
+            for (int sec = 0; sec < 2 && e == hipSuccess; ++sec)
+                if (p.func == fn[sec]) section_node[s][sec] = nodes[i];
+        }
+        if (e == hipSuccess && (count != 3 || !section_node[s][0] || !section_node[s][1]))
+            e = hipErrorInvalidValue;
+        if (e == hipSuccess) e = hipGraphInstantiate(&graph[s], g, nullptr, nullptr, 0);
+        section_grid[s][0] = section_grid[s][1] = 1;
+        return e;
+    }
+
+    // Sets section sec of slot s's graph to a grid of `blocks`.
+    hipError_t set_section_grid(int s, int sec, unsigned blocks) {
+        if (section_grid[s][sec] == blocks) return hipSuccess;
+        hipKernelNodeParams p{};
+        p.func = sec == 0 ? reinterpret_cast<void*>(&k_serve<ServePrice, 0>)
+                          : reinterpret_cast<void*>(&k_serve<ServeGreeks, 1>);
+        p.gridDim = dim3(blocks);
+        p.blockDim = dim3(kThreads);
+        p.sharedMemBytes = 0;
+        p.kernelParams = section_args[s];
+        p.extra = nullptr;
+        const hipError_t e = hipGraphExecKernelNodeSetParams(graph[s], section_node[s][sec], &p);
+        if (e == hipSuccess) section_grid[s][sec] = blocks;
         return e;
     }
 
@@ -251,26 +285,31 @@ struct HipExec {
     void thread_start() { (void)hipSetDevice(device); }
 
     // Launches slot s on the next stream: graph replay, or the stage kernel
-    // and the fused kernel of each non-empty section directly, each
-    // section's grid one block per tile; then records the slot's event.
+    // and the fused kernel of each non-empty section directly; each section's
+    // grid is one block per tile.  Then records the slot's event.
     int launch(int s, const std::vector<serve::Admitted>& reqs, std::string* err) {
         write_slot(s, reqs);
         hipStream_t st = streams[nlaunch++ & 1];
-        hipError_t e;
+        const SlotHeader* h = reinterpret_cast<const SlotHeader*>(host_slot[s]);
+        hipError_t e = hipSuccess;
         if (graph[s]) {
-            e = hipGraphLaunch(graph[s], st);
+            // an empty section keeps one block, which returns at once
+            for (int sec = 0; sec < 2 && e == hipSuccess; ++sec) {
+                const int64_t blocks = std::max<int64_t>(1, std::min(h->tiles[sec], grid_cap));
+                e = set_section_grid(s, sec, (unsigned)blocks);
+            }
+            if (e == hipSuccess) e = hipGraphLaunch(graph[s], st);
         } else {
-            const SlotHeader* h = reinterpret_cast<const SlotHeader*>(host_slot[s]);
             hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, st,
                                reinterpret_cast<const uint4*>(host_slot_dev[s]),
                                reinterpret_cast<uint4*>(dev_slot[s]));
             if (h->tiles[0] > 0)
                 hipLaunchKernelGGL((k_serve<ServePrice, 0>),
-                                   dim3((unsigned)std::min(h->tiles[0], kMaxGrid)), dim3(kThreads),
+                                   dim3((unsigned)std::min(h->tiles[0], grid_cap)), dim3(kThreads),
                                    0, st, dev_slot[s]);
             if (h->tiles[1] > 0)
                 hipLaunchKernelGGL((k_serve<ServeGreeks, 1>),
-                                   dim3((unsigned)std::min(h->tiles[1], kMaxGrid)), dim3(kThreads),
+                                   dim3((unsigned)std::min(h->tiles[1], grid_cap)), dim3(kThreads),
                                    0, st, dev_slot[s]);
             e = hipGetLastError();
         }

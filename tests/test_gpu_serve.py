@@ -107,6 +107,33 @@ def test_alone_and_in_the_mix(gpu, use_graph):
         ctx.destroy()
 
 
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_capped_grid_loops_over_tiles(gpu, use_graph):
+    """grid_blocks = 7: each section's blocks loop over the batch's tiles;
+    results stay bit-identical to the batch maps, and the graph form's section
+    grids follow each batch's size (1 tile, then hundreds)."""
+    sizes = [1, 513, 70_001, 262_144]
+    bks = [books.make_book(n, books.SEED_BASE + 70 + i) for i, n in enumerate(sizes)]
+    ctx = Ctx(gpu, 256 << 20, use_graph=use_graph, grid=7)
+    try:
+        ids = [ctx.register(b) for b in bks]
+        ref = {(i, k): standalone(gpu, bks[i], k) for i in range(len(bks)) for k in (1, 2)}
+        reqs = [(i, k) for k in (1, 2) for i in range(len(bks))] * 3
+        for i, k in reqs[:8]:      # alone
+            t = ctx.submit(ids[i], k)
+            assert ctx.wait(t) == 0
+            assert same_bits(ctx.result(t, bks[i].n, k), ref[(i, k)]), (i, k)
+            gpu.call("bs_result_release", ctx.h, t)
+        ts = [ctx.submit(ids[i], k) for i, k in reqs]   # in the mix
+        for t, (i, k) in zip(ts, reqs):
+            assert ctx.wait(t) == 0
+            assert same_bits(ctx.result(t, bks[i].n, k), ref[(i, k)]), (i, k)
+            gpu.call("bs_result_release", ctx.h, t)
+        assert ctx.stats().requests_failed == 0
+    finally:
+        ctx.destroy()
+
+
 def test_admission_by_memory(gpu):
     n = 100_000
     book = books.make_book(n, books.SEED_BASE + 50)
