@@ -128,7 +128,86 @@ def timed_passes(launch, nbooks: int, steps: int, warmup: int, streams, tag: str
     return ms
 
 
-def cpu_baseline(book_host, passes: int) -> dict:
+def reference_cpu(book, threads: int, port_prices) -> dict:
+    """Times the reference itself: oracle/_ref/blackscholes_ff, compiled from
+    the reference's blackscholes.c with -DENABLE_FF (the FastFlow ParallelFor
+    map, NUM_RUNS = 100 passes, BLOCKING_MODE), on `threads` host threads.
+    Built without the PARSEC hooks it prints no ROI time, so the ROI is timed
+    from outside: its stdout is a pty (so line-buffered), the ROI begins right
+    after it prints "Size of data:" (blackscholes.c:769-782, including the
+    map's construction as the reference's own ROI does) and ends when it
+    creates the output file (:913-925).  The output is then checked against
+    the port's prices.  Returns {} when the binary is absent."""
+    import pty
+    import select
+    import shutil
+    import subprocess
+    import tempfile
+    import pyoracle
+    exe = os.path.join(ROOT, "oracle", "_ref", "blackscholes_ff")
+    if not os.access(exe, os.X_OK):
+        return {}
+    d = tempfile.mkdtemp(prefix="bs_ref_")
+    try:
+        inp, outp = os.path.join(d, "in.txt"), os.path.join(d, "out.txt")
+        pyoracle.write_parsec_input(inp, book)
+        master, slave = pty.openpty()
+        p = subprocess.Popen([exe, str(threads), inp, outp], stdin=subprocess.DEVNULL,
+                             stdout=slave, stderr=slave, close_fds=True)
+        os.close(slave)
+        text, t_begin, t_end = b"", None, None
+        deadline = time.perf_counter() + 600
+        while time.perf_counter() < deadline:
+            if t_begin is None:
+                r, _, _ = select.select([master], [], [], 0.05)
+                if r:
+                    try:
+                        chunk = os.read(master, 4096)
+                    except OSError:            # the child closed the pty
+                        chunk = b""
+                    text += chunk
+                    if b"Size of data" in text and text.rstrip(b" ").endswith(b"\n"):
+                        t_begin = time.perf_counter()
+                if p.poll() is not None and t_begin is None:
+                    break
+            else:
+                if os.path.exists(outp):
+                    t_end = time.perf_counter()
+                    break
+                if p.poll() is not None:
+                    break
+                time.sleep(0.001)
+        rc = p.wait(timeout=600)
+        while True:                            # drain what is left on the pty
+            r, _, _ = select.select([master], [], [], 0)
+            if not r:
+                break
+            try:
+                chunk = os.read(master, 4096)
+            except OSError:
+                break
+            if not chunk:
+                break
+            text += chunk
+        os.close(master)
+        if rc != 0 or t_begin is None or t_end is None:
+            raise RuntimeError(f"reference run failed (rc {rc}): {text[-400:]!r}")
+        with open(outp) as f:
+            n_out = int(f.readline())
+            got = np.array(f.read().split(), dtype=np.float64).astype(np.float32)
+        mism = int(np.count_nonzero(got.view(np.uint32) != port_prices.view(np.uint32)))
+        sec = t_end - t_begin
+        return {"value": book.n * 100 / sec, "unit": "options/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"{book.n} options x 100 passes (NUM_RUNS), oracle/_ref/blackscholes_ff "
+                          f"(blackscholes.c with -DENABLE_FF) on {threads} threads, ROI "
+                          f"{sec:.3f} s timed from its output",
+                "outputs_equal_to_port": n_out == book.n and mism == 0}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def cpu_baseline(book_host, passes: int, ref_options: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     try:
@@ -146,13 +225,22 @@ def cpu_baseline(book_host, passes: int) -> dict:
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": rate, "unit": "options/s", "cores": used, "kind": "port",
+    port = {"value": rate, "unit": "options/s", "cores": used, "kind": "port",
             "sample": f"{book_host.n} options x {passes} passes, static blocks, "
                       f"{used} pinned threads, oracle/bs_oracle.c (bit-exact with "
                       f"blackscholes.c), {sec:.2f} s",
             "single_thread_options_per_s": one.n * 2 / sec1,
             "speedup": rate / (one.n * 2 / sec1),
             "cpu_model": model, "nproc": os.cpu_count(), "affinity": aff}
+    ref = {}
+    if ref_options > 0:
+        sample = book_host.slice(0, min(ref_options, book_host.n))
+        port_prices, _, _ = pyoracle.price_f32(sample, nthreads=threads, passes=1)
+        ref = reference_cpu(sample, threads, port_prices)
+    if not ref:
+        return port
+    ref.update({"cpu_model": model, "nproc": os.cpu_count(), "affinity": aff, "port": port})
+    return ref
 
 
 def serve_bench(result: dict) -> dict:
@@ -231,6 +319,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--books", type=int, default=4)
     ap.add_argument("--cpu-passes", type=int, default=100)
+    ap.add_argument("--cpu-ref-options", type=int, default=4_000_000,
+                    help="options in the compiled reference's timed sample (0: port only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -420,7 +510,7 @@ def main() -> int:
 
     if not args.no_cpu:
         log("bench: cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(host, args.cpu_passes)
+        result["cpu_baseline"] = cpu_baseline(host, args.cpu_passes, args.cpu_ref_options)
 
     result["launch_cpu_us_per_step"] = LAUNCH_CPU_US
     print(json.dumps(result), flush=True)

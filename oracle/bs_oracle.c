@@ -40,6 +40,7 @@
 #include <pthread.h>
 #include <sched.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -375,4 +376,24 @@ double bso_batch_f32(int64_t n, const float *s, const float *k, const float *r,
     free(blocks);
     free(tids);
     return t1 - t0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The PARSEC input format (read by blackscholes.c:701-734): N, then one line
+   "s strike r divq v t type divs DGrefval" per option, floats with 9
+   significant digits so that fscanf("%f") reads back the same float32;
+   divq, divs and DGrefval are 0, type is 'P' for puts and 'C' for calls.
+   Returns 0, or -1 if the file cannot be written. */
+int bso_write_parsec_input(const char *path, int64_t n, const float *s, const float *k,
+                           const float *r, const float *v, const float *t,
+                           const uint8_t *type)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) return -1;
+    int ok = fprintf(f, "%lld\n", (long long)n) > 0;
+    for (int64_t i = 0; i < n && ok; i++)
+        ok = fprintf(f, "%.9g %.9g %.9g 0 %.9g %.9g %c 0 0\n", (double)s[i], (double)k[i],
+                     (double)r[i], (double)v[i], (double)t[i], type[i] ? 'P' : 'C') > 0;
+    if (fclose(f) != 0) ok = 0;
+    return ok ? 0 : -1;
 }

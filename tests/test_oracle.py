@@ -164,3 +164,33 @@ def test_static_blocks_under_thread_sanitizer():
     p = subprocess.run([os.path.join(native, "oracle_pool_tsan")], capture_output=True, text=True,
                        timeout=300, env=dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66"))
     assert p.returncode == 0 and "ThreadSanitizer" not in p.stderr, p.stderr[-4000:]
+
+
+def test_c_input_writer_matches_python_writer(tmp_path):
+    """pyoracle.write_parsec_input (C) writes the same text as
+    books.write_parsec_input, ragged values and both option types included."""
+    import filecmp
+    import pyoracle
+    book = books.make_book(5003, books.SEED_BASE + 17)
+    a, b = tmp_path / "py.txt", tmp_path / "c.txt"
+    books.write_parsec_input(str(a), book)
+    pyoracle.write_parsec_input(str(b), book)
+    assert filecmp.cmp(a, b, shallow=False)
+
+
+@pytest.mark.skipif(not os.access(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..",
+                                               "oracle", "_ref", "blackscholes_ff"), os.X_OK),
+                    reason="compiled reference not built (needs the reference sources)")
+def test_reference_cpu_leg_times_the_compiled_reference(oracle):
+    """bench.py's reference leg: runs oracle/_ref/blackscholes_ff on a small
+    book, finds its ROI from its output, and its prices equal the port's."""
+    import sys
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..")
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    import bench
+    book = books.make_book(20_000, books.SEED_BASE + 18)
+    port, _, _ = oracle.price_f32(book, nthreads=2)
+    r = bench.reference_cpu(book, 2, port)
+    assert r["kind"] == "reference" and r["cores"] == 2 and r["value"] > 0
+    assert r["outputs_equal_to_port"]
