@@ -109,11 +109,11 @@ class Map {
     Map& operator=(const Map&) = delete;
 
     int run() {
-        for (int j = 0; j < passes_ && n_ > 0; ++j)
-            check(bs_price_i32type_f32(static_cast<float*>(d_[0]), static_cast<float*>(d_[1]),
-                                       static_cast<float*>(d_[2]), static_cast<float*>(d_[3]),
-                                       static_cast<float*>(d_[4]), static_cast<int32_t*>(dty_),
-                                       static_cast<float*>(dout_), n_, stream_),
+        if (n_ > 0)
+            check(bs_price_passes_f32(static_cast<float*>(d_[0]), static_cast<float*>(d_[1]),
+                                      static_cast<float*>(d_[2]), static_cast<float*>(d_[3]),
+                                      static_cast<float*>(d_[4]), dty_, 4,
+                                      static_cast<float*>(dout_), n_, passes_, stream_),
                   "Map::run");
         return 0;
     }

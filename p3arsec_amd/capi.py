@@ -79,6 +79,7 @@ _SIGS = {
     "bs_price_scalar_f32": ([_f32] * 5 + [_i32, _f32, ctypes.POINTER(_f32)], _i32),
     "bs_price_f32": ([_vp] * 7 + [_i64, _vp], _i32),
     "bs_price_i32type_f32": ([_vp] * 7 + [_i64, _vp], _i32),
+    "bs_price_passes_f32": ([_vp] * 6 + [_i32, _vp, _i64, _i32, _vp], _i32),
     "bs_price_fast_f32": ([_vp] * 7 + [_i64, _vp], _i32),
     "bs_price_f64": ([_vp] * 7 + [_i64, _vp], _i32),
     "bs_price_greeks_f32": ([_vp] * 11 + [_i64, _vp], _i32),

@@ -6,8 +6,10 @@
 // HIP kernel, and fails with BS_ERR_NODEVICE / BS_ERR_HIP when it cannot.
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "bs_internal.hpp"
 
@@ -100,6 +102,97 @@ float* scalar_slot() {
         }
     }
     return slot;
+}
+
+}  // namespace
+
+// ---- repeated passes ---------------------------------------------------------------
+namespace {
+
+// Books up to this size are bound by launch cost (a pass takes a few
+// microseconds, about what the host spends per launch): their passes replay
+// from one graph.
+constexpr int64_t kPassGraphMaxOptions = int64_t(1) << 20;
+constexpr size_t kPassGraphs = 16;
+constexpr int kPassDevices = 64;
+
+// A graph is valid for the arrays, sizes and device it was captured with, and
+// for the launch settings that shaped its kernels.
+struct PassKey {
+    const void* ptr[7];
+    int64_t n;
+    int passes, type_bytes, device;
+    int64_t launch[5];   // max_blocks, vector_width, nontemporal, pipeline, block_threads
+    bool operator==(const PassKey& o) const {
+        return memcmp(ptr, o.ptr, sizeof(ptr)) == 0 && n == o.n && passes == o.passes &&
+               type_bytes == o.type_bytes && device == o.device &&
+               memcmp(launch, o.launch, sizeof(launch)) == 0;
+    }
+};
+
+struct PassGraph {
+    PassKey key;
+    hipGraphExec_t exec;
+    hipEvent_t done;   // recorded after each replay: eviction waits for it
+};
+
+struct PassGraphCache {
+    std::mutex mu;
+    std::vector<PassGraph> lru;                     // most recently used first
+    hipStream_t capture[kPassDevices] = {};
+};
+
+PassGraphCache& pass_cache() {
+    static PassGraphCache* c = new PassGraphCache();   // lives until the process ends
+    return *c;
+}
+
+hipError_t launch_pass(const SoaIn<float>& in, const void* type, int type_bytes, float* out,
+                       int64_t n, hipStream_t st) {
+    return type_bytes == 1
+               ? launch_price_f32(in, static_cast<const uint8_t*>(type), out, n, st)
+               : launch_price_i32type_f32(in, static_cast<const int32_t*>(type), out, n, st);
+}
+
+// The cached graph for key, captured now if absent (the cache's lock held).
+PassGraph* pass_graph(PassGraphCache& c, const PassKey& key, const SoaIn<float>& in,
+                      const void* type, float* out, hipError_t* e) {
+    for (size_t i = 0; i < c.lru.size(); ++i)
+        if (c.lru[i].key == key) {
+            std::rotate(c.lru.begin(), c.lru.begin() + (std::ptrdiff_t)i,
+                        c.lru.begin() + (std::ptrdiff_t)i + 1);
+            return &c.lru.front();
+        }
+    hipStream_t& cs = c.capture[key.device];
+    if (!cs && (*e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) {
+        cs = nullptr;
+        return nullptr;
+    }
+    if ((*e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return nullptr;
+    hipError_t le = hipSuccess;
+    for (int p = 0; p < key.passes && le == hipSuccess; ++p)
+        le = launch_pass(in, type, key.type_bytes, out, key.n, cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(cs, &graph);
+    *e = le != hipSuccess ? le : ce;
+    PassGraph g{key, nullptr, nullptr};
+    if (*e == hipSuccess) *e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (*e == hipSuccess) *e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+    if (*e != hipSuccess) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        return nullptr;
+    }
+    if (c.lru.size() == kPassGraphs) {
+        PassGraph& old = c.lru.back();
+        (void)hipEventSynchronize(old.done);
+        (void)hipEventDestroy(old.done);
+        (void)hipGraphExecDestroy(old.exec);
+        c.lru.pop_back();
+    }
+    c.lru.insert(c.lru.begin(), g);
+    return &c.lru.front();
 }
 
 }  // namespace
@@ -303,6 +396,42 @@ int bs_repack_aos_f32(const void* aos, int64_t n, float* s, float* k, float* r, 
 }
 
 // ---- host-pointer forms -------------------------------------------------------------
+int bs_price_passes_f32(const float* s, const float* k, const float* r, const float* v,
+                        const float* t, const void* otype, int otype_bytes, float* price,
+                        int64_t n, int passes, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n))) return rc;
+    if (passes < 1) return set_error(BS_ERR_INVALID, "passes < 1");
+    if (otype_bytes != 1 && otype_bytes != 4)
+        return set_error(BS_ERR_INVALID, "otype_bytes must be 1 or 4");
+    if (n == 0) return BS_OK;
+    if ((rc = check_soa(s, k, r, v, t, n)) ||
+        (rc = check_device_range(otype, (size_t)n * otype_bytes, "otype")) ||
+        (rc = check_device_range(price, (size_t)n * 4, "price")))
+        return rc;
+    const SoaIn<float> in{s, k, r, v, t};
+    const hipStream_t st = as_stream(stream);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+    if (n <= kPassGraphMaxOptions && passes > 1 && dev < kPassDevices) {
+        const Config& cf = config();
+        const PassKey key{{s, k, r, v, t, otype, price}, n, passes, otype_bytes, dev,
+                          {cf.max_blocks, cf.vector_width, cf.nontemporal, cf.pipeline,
+                           cf.block_threads}};
+        PassGraphCache& c = pass_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        PassGraph* g = pass_graph(c, key, in, otype, price, &e);
+        if (!g) return hip_error(e, "bs_price_passes_f32: graph capture");
+        e = hipGraphLaunch(g->exec, st);
+        if (e == hipSuccess) e = hipEventRecord(g->done, st);
+        return done(e, "bs_price_passes_f32");
+    }
+    for (int p = 0; p < passes && e == hipSuccess; ++p)
+        e = launch_pass(in, otype, otype_bytes, price, n, st);
+    return done(e, "bs_price_passes_f32");
+}
+
 int bs_price_host_f32(const float* s, const float* k, const float* r, const float* v,
                       const float* t, const int32_t* otype, float* price, int64_t n,
                       int passes, int device) {

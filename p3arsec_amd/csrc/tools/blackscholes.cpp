@@ -111,10 +111,9 @@ int main(int argc, char** argv) {
         if ((rc = bs_stream_sync(st))) return fail("repack", rc);
         // ROI: the passes only
         const auto t0 = std::chrono::steady_clock::now();
-        for (int j = 0; j < passes; ++j)
-            if ((rc = bs_price_f32(f[0], f[1], f[2], f[3], f[4], static_cast<uint8_t*>(type),
-                                   static_cast<float*>(out), (int64_t)n, st)))
-                return fail("price", rc);
+        if ((rc = bs_price_passes_f32(f[0], f[1], f[2], f[3], f[4], type, 1,
+                                      static_cast<float*>(out), (int64_t)n, passes, st)))
+            return fail("price", rc);
         if ((rc = bs_stream_sync(st))) return fail("price", rc);
         const double roi = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         printf("roi.time|%.6f\n", roi);

@@ -64,6 +64,12 @@ def test_argument_errors_before_device_work():
     # plain host memory is never handed to a kernel
     assert L.bs_price_f32(p, p, p, p, p, p, p, 16, None) == -1
     assert L.bs_repack_aos_f32(p + 1, 4, p, p, p, p, p, p, None) == -1   # misaligned AoS
+    six = [None] * 6
+    assert L.bs_price_passes_f32(*six, 1, None, 16, 0, None) == -1      # passes < 1
+    assert L.bs_price_passes_f32(*six, 3, None, 16, 2, None) == -1      # type width 1 or 4
+    assert L.bs_price_passes_f32(*six, 1, None, 0, 100, None) == 0      # empty book
+    assert L.bs_price_passes_f32(*six, 1, None, 16, 100, None) == -1    # NULL arrays
+    assert L.bs_price_passes_f32(p, p, p, p, p, p, 1, p, 16, 2, None) == -1   # host memory
 
 
 def test_scalar_and_thread_compat_arguments():
