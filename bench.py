@@ -293,6 +293,11 @@ def only_one(args, L, host, dbooks, outs, streams) -> int:
         fn = lambda i, s: L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s)  # noqa
     elif args.only == "fast":
         fn = lambda i, s: L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s)  # noqa
+    elif args.only in ("portfolio", "portfolio_greeks"):
+        g = 1 if args.only == "portfolio_greeks" else 0
+        extra = [capi.DeviceBuffer(40) for _ in range(nb)]
+        fn = lambda i, s: L.bs_portfolio_f32(*dbooks[i].ptrs(), None, n, g,  # noqa
+                                             extra[i].ptr, s)
     elif args.only == "greeks":
         extra = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
         fn = lambda i, s: L.bs_price_greeks_f32(*dbooks[i].ptrs(),  # noqa
@@ -304,7 +309,7 @@ def only_one(args, L, host, dbooks, outs, streams) -> int:
         extra = [d64, o64]
         nb = 2
         fn = lambda i, s: L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, s)  # noqa
-    ss = streams[:1] if args.one_stream else streams[:STREAMS[args.only]]
+    ss = streams[:1] if args.one_stream else streams[:STREAMS.get(args.only, 2)]
     ms = timed_passes(lambda i, s: capi.check(fn(i, s), args.only), nb, args.steps, args.warmup,
                       ss)
     print(json.dumps({"kernel": args.only, "options": n, "ms_per_step": ms}), flush=True)
@@ -325,7 +330,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy"],
+    ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy", "portfolio",
+                                       "portfolio_greeks"],
                     help="time one kernel only (for profiler runs)")
     ap.add_argument("--one-stream", action="store_true",
                     help="with --only: every pass on one stream")
