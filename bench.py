@@ -23,6 +23,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -236,7 +237,11 @@ def cpu_baseline(book_host, passes: int, ref_options: int) -> dict:
     if ref_options > 0:
         sample = book_host.slice(0, min(ref_options, book_host.n))
         port_prices, _, _ = pyoracle.price_f32(sample, nthreads=threads, passes=1)
-        ref = reference_cpu(sample, threads, port_prices)
+        try:
+            ref = reference_cpu(sample, threads, port_prices)
+        except (RuntimeError, OSError, ValueError, subprocess.SubprocessError) as e:
+            port["reference_run_failed"] = str(e)[:300]   # the port stands in
+            ref = {}
     if not ref:
         return port
     ref.update({"cpu_model": model, "nproc": os.cpu_count(), "affinity": aff, "port": port})
