@@ -77,6 +77,13 @@ LAUNCH_CPU_US = {}
 STREAMS = {"price": 2, "fast": 2, "greeks": 1, "fp64": 2}
 
 
+def section_failed(result: dict, name: str, e: Exception) -> None:
+    """Records a failed optional section in the JSON line and on stderr."""
+    import traceback
+    traceback.print_exc()
+    result.setdefault("errors", {})[name] = f"{type(e).__name__}: {e}"[:500]
+
+
 def spin_up(seconds: float = 0.25) -> None:
     """Brings the GPU to its working clocks before anything is timed: plain
     256 MiB device-to-device copies for `seconds` (the clocks need about 30 ms
@@ -142,7 +149,6 @@ def reference_cpu(book, threads: int, port_prices) -> dict:
     import pty
     import select
     import shutil
-    import subprocess
     import tempfile
     import pyoracle
     exe = os.path.join(ROOT, "oracle", "_ref", "blackscholes_ff")
@@ -255,7 +261,6 @@ def serve_bench(result: dict) -> dict:
     saturation runs (4096 requests outstanding) in both launch forms; compares the
     saturated options/s with the batch kernels' options/s for the same 70/30
     price/Greeks mix."""
-    import subprocess
     exe = os.path.join(ROOT, "p3arsec_amd", "bin", "bs_serve_bench")
     secs = str(ARGS.serve_seconds)
     out = {}
@@ -411,123 +416,132 @@ def main() -> int:
                           "source": "profiles/hbm_traffic.json (rocprofv3 --pmc SQ_INSTS_VALU)"}
 
     if not args.no_extra:
-        def launch_fast(i, s):
-            capi.check(L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s), "fast")
-        log("bench: fast price map")
-        fast_ms = timed_passes(launch_fast, nb, args.steps, min(args.warmup, 200),
-                               streams[:STREAMS["fast"]])
-        result["fast_variant"] = {"ms_per_step": fast_ms, "options_per_s": n / (fast_ms * 1e-3),
-                                  "achieved_gbs": alg_bytes / (fast_ms * 1e-3) / 1e9}
-        gouts = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
+        try:
+            def launch_fast(i, s):
+                capi.check(L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s), "fast")
+            log("bench: fast price map")
+            fast_ms = timed_passes(launch_fast, nb, args.steps, min(args.warmup, 200),
+                                   streams[:STREAMS["fast"]])
+            result["fast_variant"] = {"ms_per_step": fast_ms, "options_per_s": n / (fast_ms * 1e-3),
+                                      "achieved_gbs": alg_bytes / (fast_ms * 1e-3) / 1e9}
+            gouts = [[capi.alloc_out(n) for _ in range(5)] for _ in range(nb)]
 
-        def launch_greeks(i, s):
-            capi.check(L.bs_price_greeks_f32(*dbooks[i].ptrs(), *[o.ptr for o in gouts[i]],
-                                             n, s), "greeks")
-        log("bench: greeks map")
-        spin_up(0.1)
-        gms = timed_passes(launch_greeks, nb, max(10, args.steps // 2), min(args.warmup, 200),
-                           streams[:STREAMS["greeks"]], "greeks")
-        result["greeks"] = {"ms_per_step": gms, "ratio_to_price": gms / ms,
-                            "ratio_to_price_one_stream": gms / ms_one,
-                            "options_per_s": n / (gms * 1e-3),
-                            "achieved_gbs": 41 * n / (gms * 1e-3) / 1e9}
-        for row in gouts:
-            for o in row:
+            def launch_greeks(i, s):
+                capi.check(L.bs_price_greeks_f32(*dbooks[i].ptrs(), *[o.ptr for o in gouts[i]],
+                                                 n, s), "greeks")
+            log("bench: greeks map")
+            spin_up(0.1)
+            gms = timed_passes(launch_greeks, nb, max(10, args.steps // 2), min(args.warmup, 200),
+                               streams[:STREAMS["greeks"]], "greeks")
+            result["greeks"] = {"ms_per_step": gms, "ratio_to_price": gms / ms,
+                                "ratio_to_price_one_stream": gms / ms_one,
+                                "options_per_s": n / (gms * 1e-3),
+                                "achieved_gbs": 41 * n / (gms * 1e-3) / 1e9}
+            for row in gouts:
+                for o in row:
+                    o.free()
+            # FP64 map: one book copy in double (2 rotating copies = 980 MB)
+            log("bench: fp64 map")
+            h64 = host.astype(np.float64)
+            d64 = [capi.DeviceBook(h64, dtype=np.float64) for _ in range(2)]
+            o64 = [capi.alloc_out(n, np.float64) for _ in range(2)]
+
+            def launch_f64(i, s):
+                capi.check(L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, s), "f64")
+            spin_up(0.1)   # the uploads above left the GPU idle
+            fms = timed_passes(launch_f64, 2, max(10, args.steps // 4), min(args.warmup, 500),
+                               streams[:STREAMS["fp64"]])
+            result["fp64"] = {"ms_per_step": fms, "ratio_to_fp32": fms / ms,
+                              "options_per_s": n / (fms * 1e-3),
+                              "achieved_gbs": 49 * n / (fms * 1e-3) / 1e9}
+            for b in d64:
+                b.free()
+            for o in o64:
                 o.free()
-        # FP64 map: one book copy in double (2 rotating copies = 980 MB)
-        log("bench: fp64 map")
-        h64 = host.astype(np.float64)
-        d64 = [capi.DeviceBook(h64, dtype=np.float64) for _ in range(2)]
-        o64 = [capi.alloc_out(n, np.float64) for _ in range(2)]
+            # portfolio sums over the same book (the MapReduce step after the map):
+            # 21 B/option read, nothing per option written
+            log("bench: portfolio sums")
+            sums = [capi.DeviceBuffer(40) for _ in range(nb)]
 
-        def launch_f64(i, s):
-            capi.check(L.bs_price_f64(*d64[i].ptrs(), o64[i].ptr, n, s), "f64")
-        spin_up(0.1)   # the uploads above left the GPU idle
-        fms = timed_passes(launch_f64, 2, max(10, args.steps // 4), min(args.warmup, 500),
-                           streams[:STREAMS["fp64"]])
-        result["fp64"] = {"ms_per_step": fms, "ratio_to_fp32": fms / ms,
-                          "options_per_s": n / (fms * 1e-3),
-                          "achieved_gbs": 49 * n / (fms * 1e-3) / 1e9}
-        for b in d64:
-            b.free()
-        for o in o64:
-            o.free()
-        # portfolio sums over the same book (the MapReduce step after the map):
-        # 21 B/option read, nothing per option written
-        log("bench: portfolio sums")
-        sums = [capi.DeviceBuffer(40) for _ in range(nb)]
-
-        def launch_pf(i, s, greeks):
-            capi.check(L.bs_portfolio_f32(*dbooks[i].ptrs(), None, n, greeks, sums[i].ptr, s),
-                       "portfolio")
-        tpf = timed_passes(lambda i, s: launch_pf(i, s, 0), nb, max(10, args.steps // 2),
-                           min(args.warmup, 200), streams[:STREAMS["price"]])
-        tpg = timed_passes(lambda i, s: launch_pf(i, s, 1), nb, max(10, args.steps // 4),
-                           min(args.warmup, 100), streams[:STREAMS["price"]])
-        result["portfolio"] = {"price_sum_us": tpf * 1e3, "price_sum_gbs": 21 * n / (tpf * 1e-3) / 1e9,
-                               "greeks_sums_us": tpg * 1e3,
-                               "greeks_sums_gbs": 21 * n / (tpg * 1e-3) / 1e9}
-        for b in sums:
-            b.free()
-        # the 4096-option book of the reference's CPU-runnable case: L2-resident, so a
-        # per-pass latency, not an HBM figure; error against the FP64 closed form
-        log("bench: 4096-option book")
-        small = books.make_book(4096, books.SEED_BASE + 1)
-        dsm, osm = capi.DeviceBook(small), capi.alloc_out(4096)
-        t4 = timed_passes(lambda i, s: capi.check(L.bs_price_f32(*dsm.ptrs(), osm.ptr, 4096, s),
-                                                  "4096"), 1, 100, 100, streams[:1])
-        # the 100 passes as one call: replayed from a captured graph
-        sp = dsm.ptrs()
-        t4g = timed_passes(lambda i, s: capi.check(L.bs_price_passes_f32(
-            *sp[:5], sp[5], 1, osm.ptr, 4096, 100, s), "4096 graph"), 1, 20, 5, streams[:1]) / 100
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        p4 = osm.download(np.float32, 4096).astype(np.float64)
-        e4 = np.abs(p4 - pyoracle.price_exact(small))
-        r4, _, _ = pyoracle.price_f32(small)
-        result["config_4096"] = {"us_per_pass": t4g * 1e3, "options_per_s": 4096 / (t4g * 1e-3),
-                                 "passes": 100, "form": "bs_price_passes_f32 (100 passes, graph replay)",
-                                 "us_per_pass_launch_by_launch": t4 * 1e3,
-                                 "max_err_vs_fp64_closed_form": float(e4.max()),
-                                 "mean_err_vs_fp64_closed_form": float(e4.mean()),
-                                 "max_abs_diff_vs_reference": float(np.abs(p4 - r4).max())}
-        dsm.free()
-        osm.free()
-        # one GPU's share of an 8-way sharded book (10M/8 options, one book, not
-        # rotated): the working set fits the 256 MB MALL, so this is not HBM-bound
-        log("bench: 1.25M-option shard")
-        ns = n // 8
-        shard = capi.DeviceBook(host.slice(0, ns))
-        oshard = capi.alloc_out(ns)
-        tsh = timed_passes(lambda i, s: capi.check(L.bs_price_f32(*shard.ptrs(), oshard.ptr, ns, s),
-                                                   "shard"), 1, 200, 200, streams[:1])
-        result["shard_1_25m"] = {"us_per_pass": tsh * 1e3, "options_per_s": ns / (tsh * 1e-3),
-                                 "note": "one book reused every pass: MALL-resident working set"}
-        shard.free()
-        oshard.free()
-        # host-buffer boundary (the SkePU2 Map form): upload, one pass, download
-        log("bench: host-buffer map")
-        arrs = [np.ascontiguousarray(a) for a in host.arrays()[:5]]
-        ty32 = host.otype.astype(np.int32)
-        hout = np.empty(n, np.float32)
-        hts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            capi.check(L.bs_price_host_f32(*[a.ctypes.data for a in arrs], ty32.ctypes.data,
-                                           hout.ctypes.data, n, 1, 0), "host")
-            hts.append(time.perf_counter() - t0)
-        result["host_buffers"] = {"seconds_per_call": min(hts),
-                                  "options_per_s": n / min(hts),
-                                  "note": "pageable host arrays in and out (PCIe-inclusive), "
-                                          "one pass; not the headline value"}
+            def launch_pf(i, s, greeks):
+                capi.check(L.bs_portfolio_f32(*dbooks[i].ptrs(), None, n, greeks, sums[i].ptr, s),
+                           "portfolio")
+            tpf = timed_passes(lambda i, s: launch_pf(i, s, 0), nb, max(10, args.steps // 2),
+                               min(args.warmup, 200), streams[:STREAMS["price"]])
+            tpg = timed_passes(lambda i, s: launch_pf(i, s, 1), nb, max(10, args.steps // 4),
+                               min(args.warmup, 100), streams[:STREAMS["price"]])
+            result["portfolio"] = {"price_sum_us": tpf * 1e3, "price_sum_gbs": 21 * n / (tpf * 1e-3) / 1e9,
+                                   "greeks_sums_us": tpg * 1e3,
+                                   "greeks_sums_gbs": 21 * n / (tpg * 1e-3) / 1e9}
+            for b in sums:
+                b.free()
+            # the 4096-option book of the reference's CPU-runnable case: L2-resident, so a
+            # per-pass latency, not an HBM figure; error against the FP64 closed form
+            log("bench: 4096-option book")
+            small = books.make_book(4096, books.SEED_BASE + 1)
+            dsm, osm = capi.DeviceBook(small), capi.alloc_out(4096)
+            t4 = timed_passes(lambda i, s: capi.check(L.bs_price_f32(*dsm.ptrs(), osm.ptr, 4096, s),
+                                                      "4096"), 1, 100, 100, streams[:1])
+            # the 100 passes as one call: replayed from a captured graph
+            sp = dsm.ptrs()
+            t4g = timed_passes(lambda i, s: capi.check(L.bs_price_passes_f32(
+                *sp[:5], sp[5], 1, osm.ptr, 4096, 100, s), "4096 graph"), 1, 20, 5, streams[:1]) / 100
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+            p4 = osm.download(np.float32, 4096).astype(np.float64)
+            e4 = np.abs(p4 - pyoracle.price_exact(small))
+            r4, _, _ = pyoracle.price_f32(small)
+            result["config_4096"] = {"us_per_pass": t4g * 1e3, "options_per_s": 4096 / (t4g * 1e-3),
+                                     "passes": 100, "form": "bs_price_passes_f32 (100 passes, graph replay)",
+                                     "us_per_pass_launch_by_launch": t4 * 1e3,
+                                     "max_err_vs_fp64_closed_form": float(e4.max()),
+                                     "mean_err_vs_fp64_closed_form": float(e4.mean()),
+                                     "max_abs_diff_vs_reference": float(np.abs(p4 - r4).max())}
+            dsm.free()
+            osm.free()
+            # one GPU's share of an 8-way sharded book (10M/8 options, one book, not
+            # rotated): the working set fits the 256 MB MALL, so this is not HBM-bound
+            log("bench: 1.25M-option shard")
+            ns = n // 8
+            shard = capi.DeviceBook(host.slice(0, ns))
+            oshard = capi.alloc_out(ns)
+            tsh = timed_passes(lambda i, s: capi.check(L.bs_price_f32(*shard.ptrs(), oshard.ptr, ns, s),
+                                                       "shard"), 1, 200, 200, streams[:1])
+            result["shard_1_25m"] = {"us_per_pass": tsh * 1e3, "options_per_s": ns / (tsh * 1e-3),
+                                     "note": "one book reused every pass: MALL-resident working set"}
+            shard.free()
+            oshard.free()
+            # host-buffer boundary (the SkePU2 Map form): upload, one pass, download
+            log("bench: host-buffer map")
+            arrs = [np.ascontiguousarray(a) for a in host.arrays()[:5]]
+            ty32 = host.otype.astype(np.int32)
+            hout = np.empty(n, np.float32)
+            hts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                capi.check(L.bs_price_host_f32(*[a.ctypes.data for a in arrs], ty32.ctypes.data,
+                                               hout.ctypes.data, n, 1, 0), "host")
+                hts.append(time.perf_counter() - t0)
+            result["host_buffers"] = {"seconds_per_call": min(hts),
+                                      "options_per_s": n / min(hts),
+                                      "note": "pageable host arrays in and out (PCIe-inclusive), "
+                                              "one pass; not the headline value"}
+        except Exception as e:  # recorded, so the headline line still prints
+            section_failed(result, "extras", e)
 
     if not args.no_extra and not args.no_serve:
-        log("bench: serving loop")
-        result["serve"] = serve_bench(result)
+        try:
+            log("bench: serving loop")
+            result["serve"] = serve_bench(result)
+        except Exception as e:  # recorded, so the headline line still prints
+            section_failed(result, "serve", e)
 
     if not args.no_cpu:
-        log("bench: cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(host, args.cpu_passes, args.cpu_ref_options)
+        try:
+            log("bench: cpu baseline")
+            result["cpu_baseline"] = cpu_baseline(host, args.cpu_passes, args.cpu_ref_options)
+        except Exception as e:  # recorded, so the headline line still prints
+            section_failed(result, "cpu_baseline", e)
 
     result["launch_cpu_us_per_step"] = LAUNCH_CPU_US
     print(json.dumps(result), flush=True)
