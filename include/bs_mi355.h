@@ -222,7 +222,11 @@ typedef struct {
                                    512-option tile; blocks loop past the
                                    cap), 0 = 2^20                            */
     int32_t completion_queue;   /* 1: finished requests queue for bs_reap    */
-    int32_t reserved;           /* 0                                         */
+    int32_t small_request_options; /* requests of at most this many options
+                                   form batches of their own, on a stream of
+                                   their own, so they do not wait for large
+                                   requests' batches; 0 = 65536, -1 = one
+                                   lane for all requests                     */
 } bs_serve_opts;
 
 /* One finished request from the completion queue.  Times are steady-clock

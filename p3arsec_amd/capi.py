@@ -49,7 +49,7 @@ _f32 = ctypes.c_float
 class ServeOpts(ctypes.Structure):
     _fields_ = [("max_batch_requests", _i64), ("max_batch_options", _i64),
                 ("use_graph", ctypes.c_int32), ("grid_blocks", ctypes.c_int32),
-                ("completion_queue", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("completion_queue", ctypes.c_int32), ("small_request_options", ctypes.c_int32)]
 
 
 class Completion(ctypes.Structure):

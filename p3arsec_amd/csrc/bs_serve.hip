@@ -4,7 +4,7 @@
 // A context owns
 //   * one device arena (hipMalloc of the HBM budget) holding resident books
 //     and request outputs, carved by serve::Core's segregated-fit allocator;
-//   * four descriptor slots, each a pinned host table plus its device copy;
+//   * six descriptor slots, each a pinned host table plus its device copy;
 //   * per batch, a stage kernel that copies the slot's table from pinned host
 //     memory to the device, then one fused kernel per section (the batch's
 //     price requests, its Greeks requests): a block prices one tile of 512
@@ -15,10 +15,13 @@
 //     the batch's grid (hipGraphExecKernelNodeSetParams).  On ROCm 7.2 the
 //     direct form measured 15 us launch-to-completion against 31 us;
 //   * a dispatcher thread (serve_loop.hpp): it admits queued requests as arena
-//     memory allows, writes a table into a free slot and launches it, batches
-//     alternating over two streams (up to four in flight), and retires each
-//     batch when its event completes.  Requests that arrive while batches run
-//     go into the next one.
+//     memory allows, writes a table into a free slot and launches it, and
+//     retires each batch when its event completes.  Requests of at most 64K
+//     options (bs_serve_opts.small_request_options) form batches of their own
+//     on a stream of their own (two in flight), so they complete beside large
+//     requests rather than behind them; the rest alternate over two streams
+//     (four in flight).  Requests that arrive while batches run go into the
+//     next batch of their lane.
 // Results are bit-identical to bs_price_f32 / bs_price_greeks_f32: the fused
 // kernels run the batch map's own per-group code (bs_ops.hpp).
 #include <stddef.h>
@@ -138,19 +141,23 @@ using ServeGreeks = GreeksF32<kVW>;
 }  // namespace
 
 // The HIP executor of the serving loop (serve_loop.hpp): descriptor tables,
-// the stage and fused kernels, two streams and one event per slot.
+// the stage and fused kernels, three streams and one event per slot.
 struct HipExec {
-    // Batches in flight at once.  While one runs, the next ones are queued
-    // behind it, so the GPU stays busy while clients turn finished requests
-    // around into new ones.
-    static constexpr int kSlots = 4;
+    // Batches in flight at once: two of the small lane on their own stream,
+    // four of the bulk lane alternating over two streams.  While one runs, the
+    // next ones are queued behind it, so the GPU stays busy while clients turn
+    // finished requests around into new ones; the small lane's batches run
+    // beside the bulk lane's instead of behind them.
+    static constexpr int kSlots = 6;
+    static constexpr int kSmallSlots = 2;
 
     int device = 0;
     int use_graph = 0;
     int64_t grid_cap = kMaxGrid;                // blocks per section at most
     char* arena = nullptr;
     hipStream_t stream = nullptr;               // capture stream
-    hipStream_t streams[2] = {nullptr, nullptr};  // batches alternate over these
+    hipStream_t streams[3] = {};                // bulk batches alternate over 0 and 1;
+                                                // the small lane's run on 2
     uint8_t* host_slot[kSlots] = {};            // pinned, written by the dispatcher
     uint8_t* host_slot_dev[kSlots] = {};        // the same memory, device address
     uint8_t* dev_slot[kSlots] = {};
@@ -168,7 +175,7 @@ struct HipExec {
         hipError_t e = hipSetDevice(device);
         if (e == hipSuccess) e = hipMalloc((void**)&arena, arena_bytes);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-        for (int j = 0; j < 2 && e == hipSuccess; ++j)
+        for (int j = 0; j < 3 && e == hipSuccess; ++j)
             e = hipStreamCreateWithFlags(&streams[j], hipStreamNonBlocking);
         for (int s = 0; s < kSlots && e == hipSuccess; ++s) {
             e = hipHostMalloc((void**)&host_slot[s], kSlotBytes, hipHostMallocCoherent);
@@ -284,12 +291,13 @@ struct HipExec {
     // ---- the executor interface ----
     void thread_start() { (void)hipSetDevice(device); }
 
-    // Launches slot s on the next stream: graph replay, or the stage kernel
+    // Launches slot s on its lane's stream (the bulk lane alternates between
+    // two): graph replay, or the stage kernel
     // and the fused kernel of each non-empty section directly; each section's
     // grid is one block per tile.  Then records the slot's event.
     int launch(int s, const std::vector<serve::Admitted>& reqs, std::string* err) {
         write_slot(s, reqs);
-        hipStream_t st = streams[nlaunch++ & 1];
+        hipStream_t st = s < kSmallSlots ? streams[2] : streams[nlaunch++ & 1];
         const SlotHeader* h = reinterpret_cast<const SlotHeader*>(host_slot[s]);
         hipError_t e = hipSuccess;
         if (graph[s]) {
@@ -367,8 +375,11 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
     serve::Limits lim;
     lim.max_batch_requests = kMaxDesc;
     lim.max_batch_options = int64_t(1) << 31;
+    lim.small_options = 65536;
     int use_graph = 0, grid_blocks = 0;
     if (opts) {
+        if (opts->small_request_options != 0)
+            lim.small_options = opts->small_request_options < 0 ? 0 : opts->small_request_options;
         if (opts->max_batch_requests > 0)
             lim.max_batch_requests = std::min<int64_t>(opts->max_batch_requests, kMaxDesc);
         if (opts->max_batch_options > 0) lim.max_batch_options = opts->max_batch_options;

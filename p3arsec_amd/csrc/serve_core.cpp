@@ -264,9 +264,10 @@ int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket, int64_t u
     r->user = user;
     r->kind = kind;
     r->n = b->n;
+    r->lane = lane_of(r->n);
     r->t_submit = now;
-    pending_.push_back(r->ticket);
-    pending_options_ += r->n;
+    pending_[r->lane].push_back(r->ticket);
+    pending_options_[r->lane] += r->n;
     ++b->refs;
     ++st_.requests_queued;
     *ticket = r->ticket;
@@ -291,8 +292,9 @@ int Core::cancel(int64_t ticket, int64_t now) {
     Request* r = slot_of(ticket);
     if (!r) return kErrNotFound;
     if (r->state != kQueued) return kErrBusy;
-    pending_.erase(std::find(pending_.begin(), pending_.end(), ticket));
-    pending_options_ -= r->n;
+    std::deque<int64_t>& q = pending_[r->lane];
+    q.erase(std::find(q.begin(), q.end(), ticket));
+    pending_options_[r->lane] -= r->n;
     r->state = kCancelled;
     r->status = kErrCancelled;
     r->t_done = now;
@@ -303,13 +305,15 @@ int Core::cancel(int64_t ticket, int64_t now) {
     return kOk;
 }
 
-size_t Core::admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap) {
+size_t Core::admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap, int lane) {
     out->clear();
+    if (lane != kLaneSmall && lane != kLaneBulk) return 0;
+    std::deque<int64_t>& pending = pending_[lane];
     int64_t options = 0;
     const int64_t cap =
         option_cap > 0 ? std::min(option_cap, lim_.max_batch_options) : lim_.max_batch_options;
-    while (!pending_.empty() && (int64_t)out->size() < lim_.max_batch_requests) {
-        Request* r = slot_of(pending_.front());
+    while (!pending.empty() && (int64_t)out->size() < lim_.max_batch_requests) {
+        Request* r = slot_of(pending.front());
         if (!out->empty() && options + r->n > cap) break;
         size_t o;
         const int64_t h = arena_.alloc(output_bytes(r->n, r->kind), &o);
@@ -317,8 +321,8 @@ size_t Core::admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap) 
             ++st_.admission_waits;  // strict FIFO: wait for memory
             break;
         }
-        pending_.pop_front();
-        pending_options_ -= r->n;
+        pending.pop_front();
+        pending_options_[lane] -= r->n;
         r->out_off = o;
         r->out_blk = h;
         r->state = kRunning;
@@ -358,17 +362,19 @@ void Core::complete(const std::vector<int64_t>& tickets, bool ok, int64_t now) {
 }
 
 void Core::fail_queued(int status, int64_t now) {
-    while (!pending_.empty()) {
-        Request* r = slot_of(pending_.front());
-        pending_.pop_front();
-        pending_options_ -= r->n;
-        r->state = kFailed;
-        r->status = status;
-        r->t_done = now;
-        r->bookp->refs--;
-        --st_.requests_queued;
-        ++st_.requests_failed;
-        finished(*r);
+    for (int lane = 0; lane < kLanes; ++lane) {
+        while (!pending_[lane].empty()) {
+            Request* r = slot_of(pending_[lane].front());
+            pending_[lane].pop_front();
+            pending_options_[lane] -= r->n;
+            r->state = kFailed;
+            r->status = status;
+            r->t_done = now;
+            r->bookp->refs--;
+            --st_.requests_queued;
+            ++st_.requests_failed;
+            finished(*r);
+        }
     }
 }
 

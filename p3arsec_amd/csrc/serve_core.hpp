@@ -79,6 +79,9 @@ enum : int {
 };
 enum : int { kPrice = 1, kGreeks = 2 };
 enum : int { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kCancelled = 4 };
+// Admission lanes: requests of at most Limits::small_options options queue in
+// the small lane and form batches of their own; the rest in the bulk lane.
+enum : int { kLaneSmall = 0, kLaneBulk = 1, kLanes = 2 };
 
 struct BookRec {
     int64_t id = 0;
@@ -95,6 +98,7 @@ struct Request {
     int64_t user = 0;
     int kind = kPrice;
     int64_t n = 0;
+    int lane = kLaneBulk;
     int state = kQueued;
     int status = kOk;
     size_t out_off = 0;
@@ -126,6 +130,7 @@ struct Completion {
 struct Limits {
     int64_t max_batch_requests = 4096;
     int64_t max_batch_options = int64_t(1) << 30;
+    int64_t small_options = 0;  // 0: one lane (every request in the bulk lane)
 };
 
 struct Stats {
@@ -158,12 +163,13 @@ class Core {
     int submit_many(const int64_t* books, const int* kinds, const int64_t* users, int64_t count,
                     int64_t now, int64_t* tickets);
     int cancel(int64_t ticket, int64_t now);
-    // Admits queued requests in FIFO order while their outputs fit in the
-    // arena and the batch limits allow; admitted requests become Running and
-    // are appended to *out (cleared first).  option_cap > 0 lowers the
-    // batch's option limit for this call (the first request is always
-    // admitted if it fits).  Returns the number admitted.
-    size_t admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap = 0);
+    // Admits the lane's queued requests in FIFO order while their outputs fit
+    // in the arena and the batch limits allow; admitted requests become
+    // Running and are appended to *out (cleared first).  option_cap > 0
+    // lowers the batch's option limit for this call (the first request is
+    // always admitted if it fits).  Returns the number admitted.
+    size_t admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap = 0,
+                 int lane = kLaneBulk);
     // Marks a launched batch finished.  Failed requests release their outputs.
     void complete(const std::vector<int64_t>& tickets, bool ok, int64_t now);
     // Fails every queued request (shutdown).
@@ -179,8 +185,12 @@ class Core {
     int64_t reap(Completion* out, int64_t max);
     size_t completions_pending() const { return completed_.size(); }
 
-    size_t queued() const { return pending_.size(); }
-    int64_t queued_options() const { return pending_options_; }
+    size_t queued() const { return pending_[0].size() + pending_[1].size(); }
+    size_t queued(int lane) const { return pending_[lane].size(); }
+    int64_t queued_options(int lane = kLaneBulk) const { return pending_options_[lane]; }
+    int lane_of(int64_t n) const {
+        return lim_.small_options > 0 && n <= lim_.small_options ? kLaneSmall : kLaneBulk;
+    }
     const Arena& arena() const { return arena_; }
     Stats stats() const;
 
@@ -204,9 +214,9 @@ class Core {
     std::vector<std::unique_ptr<Request[]>> chunks_;  // stable request storage
     std::vector<int32_t> spare_slots_;
     int64_t nslots_ = 0;
-    std::deque<int64_t> pending_;
+    std::deque<int64_t> pending_[kLanes];
     std::deque<int64_t> completed_;
-    int64_t pending_options_ = 0;
+    int64_t pending_options_[kLanes] = {0, 0};
     bool cq_ = false;
     Stats st_;
 };

@@ -10,18 +10,27 @@
 //
 // Executor interface:
 //   static constexpr int kSlots;                       batches in flight at most
+//   static constexpr int kSmallSlots;                  slots [0, kSmallSlots) carry the
+//                                                      small lane's batches, the rest
+//                                                      the bulk lane's
 //   void thread_start();                               on the dispatcher thread
 //   int launch(int slot, const std::vector<Admitted>&, std::string* err);   0 = ok
 //   int query(int slot, std::string* err);             0 running, 1 done, -1 failed
 //   void drain();                                      waits for every launched batch
+//
+// Lanes: with Limits::small_options > 0, requests of at most that many
+// options queue in the small lane and are batched only with each other, in
+// the small lane's slots, so their completion does not wait for a large
+// request's batch; the executor runs the two lanes on different streams.
+// Each lane admits in FIFO order.
 //
 // Locking: `mu_` guards the Core, `fatal_` and `batch_ns_`.  The dispatcher
 // takes it only to admit a batch or retire one; while batches run it queries
 // them with the lock released, so submitting and reaping threads never wait
 // behind the poll loop.  `gen_` counts the changes that can unblock admission
 // (a submit, a freed result or book, a cancel, shutdown); `queued_` mirrors
-// the queue length; `finished_` counts finished requests (reapers spin on
-// it).  All three are written under `mu_` and read without it.
+// each lane's queue length; `finished_` counts finished requests (reapers
+// spin on it).  All three are written under `mu_` and read without it.
 #pragma once
 #include <stdint.h>
 
@@ -53,9 +62,10 @@ struct LoopTimes {
 template <class Exec>
 class Loop {
    public:
-    // With fewer than two batches in flight and more than this many options
-    // queued, the dispatcher admits half of the queue so that the other half's
-    // launch (and the clients' turnaround of the first half) overlaps the first.
+    // With fewer than two bulk batches in flight and more than this many
+    // options queued in the bulk lane, the dispatcher admits half of that
+    // queue so that the other half's launch (and the clients' turnaround of
+    // the first half) overlaps the first.
     static constexpr int64_t kSplitOptions = int64_t(4) << 20;
     // A reaper with nothing to reap spins this long on `finished_` before it
     // sleeps: under load a batch finishes every few hundred microseconds, and a
@@ -272,7 +282,8 @@ class Loop {
 
     // Called under `mu_` after any change to the queue or the arena.
     void changed() {
-        queued_.store((int64_t)core_.queued(), std::memory_order_release);
+        for (int lane = 0; lane < kLanes; ++lane)
+            queued_[lane].store((int64_t)core_.queued(lane), std::memory_order_release);
         gen_.fetch_add(1, std::memory_order_acq_rel);
     }
 
@@ -292,10 +303,12 @@ class Loop {
     }
 
     void dispatcher() {
+        static_assert(Exec::kSmallSlots >= 0 && Exec::kSmallSlots < Exec::kSlots,
+                      "the bulk lane needs a slot");
         ex_->thread_start();
         std::vector<Batch> inflight;        // batches on the device
         bool busy[Exec::kSlots] = {};
-        int64_t blocked_gen = -1;           // gen at the last admission that found no room
+        int64_t blocked_gen[kLanes] = {-1, -1};   // gen at the lane's last admission that found no room
         int64_t t_empty = -1;               // when the last batch in flight retired
         std::vector<Admitted> reqs;         // reused: no per-batch allocation
         std::string err;
@@ -316,45 +329,50 @@ class Loop {
                 const int64_t t2 = steady_ns();
                 t_retire_.fetch_add(t2 - t, std::memory_order_relaxed);
                 if (inflight.empty()) t_empty = t2;
-                blocked_gen = -1;   // a failed batch frees its outputs
+                blocked_gen[0] = blocked_gen[1] = -1;   // a failed batch frees its outputs
                 progressed = true;
             }
-            // launch the next batch into a free slot
-            int s = -1;
-            for (int j = 0; j < Exec::kSlots && s < 0; ++j)
-                if (!busy[j]) s = j;
-            if (s >= 0 && !shutdown_.load(std::memory_order_acquire) &&
-                queued_.load(std::memory_order_acquire) > 0 &&
-                gen_.load(std::memory_order_acquire) != blocked_gen) {
+            // launch the next batch of each lane into a free slot of that lane
+            for (int lane = 0; lane < kLanes; ++lane) {
+                const int lo = lane == kLaneSmall ? 0 : Exec::kSmallSlots;
+                const int hi = lane == kLaneSmall ? Exec::kSmallSlots : Exec::kSlots;
+                int s = -1;
+                for (int j = lo; j < hi && s < 0; ++j)
+                    if (!busy[j]) s = j;
+                if (s < 0 || shutdown_.load(std::memory_order_acquire) ||
+                    queued_[lane].load(std::memory_order_acquire) == 0 ||
+                    gen_.load(std::memory_order_acquire) == blocked_gen[lane])
+                    continue;
+                int bulk_inflight = 0;
+                for (const Batch& b : inflight) bulk_inflight += b.slot >= Exec::kSmallSlots;
                 const int64_t ta = steady_ns();
                 {
                     std::lock_guard<std::mutex> lk(mu_);
                     const int64_t g = gen_.load(std::memory_order_relaxed);
-                    const int64_t qo = core_.queued_options();
-                    core_.admit(steady_ns(), &reqs,
-                                inflight.size() < 2 && qo > kSplitOptions ? qo / 2 : 0);
-                    if (reqs.empty()) blocked_gen = g;   // the queue head does not fit
+                    const int64_t qo = core_.queued_options(lane);
+                    const bool split = lane == kLaneBulk && bulk_inflight < 2 && qo > kSplitOptions;
+                    core_.admit(steady_ns(), &reqs, split ? qo / 2 : 0, lane);
+                    if (reqs.empty()) blocked_gen[lane] = g;   // the lane's head does not fit
                     else changed();
                 }
-                if (!reqs.empty()) {
-                    Batch b{s, steady_ns(), {}};
-                    b.tickets.reserve(reqs.size());
-                    for (const auto& r : reqs) b.tickets.push_back(r.ticket);
-                    t_admit_.fetch_add(b.t_launch - ta, std::memory_order_relaxed);
-                    if (inflight.empty() && t_empty >= 0) {
-                        t_idle_.fetch_add(b.t_launch - t_empty, std::memory_order_relaxed);
-                        idle_launches_.fetch_add(1, std::memory_order_relaxed);
-                    }
-                    const int e = ex_->launch(s, reqs, &err);
-                    t_launch_.fetch_add(steady_ns() - b.t_launch, std::memory_order_relaxed);
-                    if (e != 0) {
-                        finish_batch(b.tickets, false, "launch failed: " + err, -1);
-                    } else {
-                        busy[s] = true;
-                        inflight.push_back(std::move(b));
-                    }
-                    progressed = true;
+                if (reqs.empty()) continue;
+                Batch b{s, steady_ns(), {}};
+                b.tickets.reserve(reqs.size());
+                for (const auto& r : reqs) b.tickets.push_back(r.ticket);
+                t_admit_.fetch_add(b.t_launch - ta, std::memory_order_relaxed);
+                if (inflight.empty() && t_empty >= 0) {
+                    t_idle_.fetch_add(b.t_launch - t_empty, std::memory_order_relaxed);
+                    idle_launches_.fetch_add(1, std::memory_order_relaxed);
                 }
+                const int e = ex_->launch(s, reqs, &err);
+                t_launch_.fetch_add(steady_ns() - b.t_launch, std::memory_order_relaxed);
+                if (e != 0) {
+                    finish_batch(b.tickets, false, "launch failed: " + err, -1);
+                } else {
+                    busy[s] = true;
+                    inflight.push_back(std::move(b));
+                }
+                progressed = true;
             }
             if (progressed) continue;
             if (!inflight.empty()) {
@@ -364,8 +382,11 @@ class Loop {
             if (shutdown_.load(std::memory_order_acquire)) break;
             std::unique_lock<std::mutex> lk(mu_);
             cv_work_.wait(lk, [&] {
-                return shutdown_.load(std::memory_order_relaxed) ||
-                       (core_.queued() > 0 && gen_.load(std::memory_order_relaxed) != blocked_gen);
+                if (shutdown_.load(std::memory_order_relaxed)) return true;
+                const int64_t g = gen_.load(std::memory_order_relaxed);
+                for (int lane = 0; lane < kLanes; ++lane)
+                    if (core_.queued(lane) > 0 && g != blocked_gen[lane]) return true;
+                return false;
             });
         }
         ex_->drain();
@@ -386,7 +407,7 @@ class Loop {
     std::string fatal_;
     int64_t batch_ns_ = 0;              // sum of launch -> completion seen, over batches
     std::atomic<int64_t> gen_{0};
-    std::atomic<int64_t> queued_{0};
+    std::atomic<int64_t> queued_[kLanes] = {};
     std::atomic<int64_t> finished_{0};
     std::atomic<int64_t> t_admit_{0}, t_launch_{0}, t_retire_{0}, t_idle_{0}, idle_launches_{0};
     std::atomic<bool> shutdown_{false};

@@ -5,7 +5,8 @@
 // benchmark), then either
 //   * open loop: Poisson arrivals at --rate requests/s for --duration-s,
 //     each request a random book, BS_REQ_GREEKS with probability
-//     --greeks-frac, else BS_REQ_PRICE; or
+//     --greeks-frac, else BS_REQ_PRICE; arrivals already due when the client
+//     gets to them go in one bs_submit_many call (at most 1024); or
 //   * --saturate: closed loop keeping --inflight requests outstanding, to
 //     measure the loop's throughput ceiling.
 // Latency = scheduled arrival -> completion observed by the dispatcher (the
@@ -47,7 +48,7 @@ void die(const char* what, int rc) {
 
 struct Opts {
     double rate = 20000.0, duration_s = 5.0, greeks_frac = 0.3, budget_gb = 32.0;
-    int books = 256, inflight = 2048, use_graph = 0;
+    int books = 256, inflight = 2048, use_graph = 0, lane_options = 0;
     int64_t max_book = 1 << 20, small = 65536;
     uint64_t seed = 20261207;
     bool saturate = false;
@@ -83,6 +84,7 @@ int main(int argc, char** argv) {
         else if ((v = val("--max-book"))) o.max_book = atoll(v);
         else if ((v = val("--seed"))) o.seed = strtoull(v, nullptr, 10);
         else if ((v = val("--use-graph"))) o.use_graph = atoi(v);
+        else if ((v = val("--small-options"))) o.lane_options = atoi(v);
         else if (strcmp(argv[i], "--saturate") == 0) o.saturate = true;
         else {
             fprintf(stderr, "unknown argument %s\n", argv[i]);
@@ -93,6 +95,7 @@ int main(int argc, char** argv) {
     bs_serve_opts so{};
     so.use_graph = o.use_graph;
     so.completion_queue = 1;
+    so.small_request_options = o.lane_options;
     bs_ctx_t ctx = nullptr;
     CHECK(bs_ctx_create(0, (size_t)(o.budget_gb * (double)(1ull << 30)), &so, &ctx));
 
@@ -167,44 +170,51 @@ int main(int argc, char** argv) {
     std::uniform_int_distribution<int> pick(0, (int)books.size() - 1);
     const int64_t t0 = now_ns();
     const int64_t t_end = t0 + (int64_t)(o.duration_s * 1e9);
-    double next = (double)t0;
+    double next = (double)t0 + gap(rng) * 1e9;   // the next scheduled arrival (open loop)
     int64_t submitted = 0;
     std::vector<bs_book_t> ids;
     std::vector<int> kinds;
     std::vector<int64_t> tags;
     std::vector<bs_ticket_t> tks;
+    auto add = [&](int64_t arrival) {
+        ids.push_back(books[pick(rng)].id);
+        kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
+        tags.push_back(arrival);
+    };
     for (;;) {
         ids.clear();
         kinds.clear();
         tags.clear();
-        int64_t arrival;
-        int64_t want = 1;
         if (o.saturate) {
             // top the window up to --inflight in one call
             int64_t room;
             const int64_t w0 = now_ns();
             while ((room = o.inflight - outstanding.load()) <= 0) std::this_thread::yield();
-            arrival = now_ns();
+            const int64_t arrival = now_ns();
             s_wait += arrival - w0;
             if (arrival >= t_end) break;
-            want = std::min<int64_t>(room, 1024);
+            for (int64_t i = 0; i < std::min<int64_t>(room, 1024); ++i) add(arrival);
         } else {
-            next += gap(rng) * 1e9;
-            arrival = (int64_t)next;   // latency counts from the scheduled arrival
-            if (arrival >= t_end) break;
-            while (now_ns() < arrival) {
+            if (next >= (double)t_end) break;
+            const int64_t w0 = now_ns();
+            while ((double)now_ns() < next) {
+            }
+            s_wait += now_ns() - w0;
+            // every arrival already due goes in this call, each tagged with its
+            // own scheduled time (latency counts from there)
+            const double now = (double)now_ns();
+            while (next <= now && next < (double)t_end && ids.size() < 1024) {
+                add((int64_t)next);
+                next += gap(rng) * 1e9;
             }
         }
-        for (int64_t i = 0; i < want; ++i) {
-            ids.push_back(books[pick(rng)].id);
-            kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
-            tags.push_back(arrival);
-        }
+        const int64_t want = (int64_t)ids.size();
         tks.resize(ids.size());
         outstanding.fetch_add(want);
+        const int64_t c0 = now_ns();
         CHECK(bs_submit_many(ctx, ids.data(), kinds.data(), tags.data(), want, tks.data()));
+        s_work += now_ns() - c0;
         submitted += want;
-        s_work += now_ns() - arrival;
     }
     done_submitting.store(true);
     completer.join();
@@ -218,7 +228,8 @@ int main(int argc, char** argv) {
            "\"p50_ms_le64k\": %.4f, \"p99_ms_le64k\": %.4f, \"max_ms\": %.4f, "
            "\"launches\": %lld, \"mean_requests_per_launch\": %.2f, \"books\": %d, "
            "\"catalog_options\": %lld, \"greeks_frac\": %.2f, \"arena_peak_bytes\": %lld, "
-           "\"admission_waits\": %lld, \"use_graph\": %d, \"mean_queue_us\": %.2f, "
+           "\"admission_waits\": %lld, \"use_graph\": %d, \"small_request_options\": %d, "
+           "\"mean_queue_us\": %.2f, "
            "\"mean_batch_us\": %.2f, \"mean_options_per_launch\": %.0f, "
            "\"dispatch_ms\": {\"admit\": %.1f, \"launch\": %.1f, \"retire\": %.1f, "
            "\"gpu_idle\": %.1f, \"idle_launches\": %lld}, "
@@ -231,7 +242,7 @@ int main(int argc, char** argv) {
            (long long)st.launches,
            st.launches ? (double)requests_done / (double)st.launches : 0.0, o.books,
            (long long)catalog_options, o.greeks_frac, (long long)st.arena_peak,
-           (long long)st.admission_waits, o.use_graph,
+           (long long)st.admission_waits, o.use_graph, o.lane_options,
            st.requests_done ? (double)st.queue_ns_total * 1e-3 / (double)st.requests_done : 0.0,
            st.launches ? (double)st.batch_ns_total * 1e-3 / (double)st.launches : 0.0,
            st.launches ? (double)st.options_priced / (double)st.launches : 0.0,

@@ -46,6 +46,23 @@ int sc_admit_cap(void* h, int64_t now, int64_t cap, int64_t* tickets, int cn) {
     return i;
 }
 int64_t sc_queued_options(void* h) { return static_cast<Core*>(h)->queued_options(); }
+// Two lanes: requests of at most small_options options queue in lane 0.
+void* sc_create_lanes(int64_t arena_bytes, int64_t small_options) {
+    bs::serve::Limits lim;
+    lim.small_options = small_options;
+    return new Core((size_t)arena_bytes, lim);
+}
+int sc_admit_lane(void* h, int64_t now, int lane, int64_t* tickets, int cn) {
+    std::vector<bs::serve::Admitted> b;
+    static_cast<Core*>(h)->admit(now, &b, 0, lane);
+    int i = 0;
+    for (const auto& r : b) {
+        if (i < cn) tickets[i] = r.ticket;
+        ++i;
+    }
+    return i;
+}
+int64_t sc_queued_lane(void* h, int lane) { return (int64_t)static_cast<Core*>(h)->queued(lane); }
 void sc_enable_completions(void* h, int on) { static_cast<Core*>(h)->enable_completions(on != 0); }
 // Pops up to max completions into parallel arrays; returns the count.
 int64_t sc_reap(void* h, int64_t* tickets, int* states, int* statuses, int64_t* users,

@@ -37,12 +37,14 @@ std::atomic<int> g_fail{0};
     } while (0)
 
 struct HostExec {
-    static constexpr int kSlots = 4;
+    static constexpr int kSlots = 6;
+    static constexpr int kSmallSlots = 2;
     size_t arena = 0;
     int fail_launch_every = 0, fail_batch_every = 0;
     int64_t fixed_ns = 20000;       // per batch
     double ns_per_option = 0.002;   // ~ 5e11 options/s
     int64_t launches = 0, batches = 0;
+    int64_t small_options = 0;   // the loop's lane threshold, checked per launch
     struct Slot {
         bool used = false, fail = false;
         int64_t done_at = 0;
@@ -67,6 +69,9 @@ struct HostExec {
         {
             std::lock_guard<std::mutex> lk(books_mu);
             for (const Admitted& r : reqs) {
+                // a batch holds one lane's requests, in that lane's slots
+                const bool small = small_options > 0 && r.n <= small_options;
+                CHECK(small == (s < kSmallSlots));
                 const size_t ob = output_bytes(r.n, r.kind);
                 CHECK(r.out_off % kAlign == 0 && r.out_off + ob <= arena);
                 CHECK(r.book_off + book_bytes(r.book_n) <= arena && r.n == r.book_n);
@@ -120,6 +125,7 @@ struct World {
 
     World(size_t arena, Limits lim, bool cq, int nbooks, uint64_t seed, int64_t max_n = 1 << 16) {
         ex.arena = arena & ~(kAlign - 1);
+        ex.small_options = lim.small_options;
         loop = new Loop<HostExec>(&ex, arena, lim, cq);
         std::mt19937_64 rng(seed);
         for (int b = 0; b < nbooks; ++b) {
@@ -205,6 +211,7 @@ void check_drained(World& w) {
 
 int scenario_stress() {
     Limits lim;
+    lim.small_options = 16384;   // about a quarter of the books take the small lane
     World w((size_t)1 << 32, lim, false, 64, 11);
     int64_t failed = 0, cancelled = 0;
     const int64_t fin = run_clients(w, 6, 150, 5, &failed, &cancelled);
@@ -222,6 +229,7 @@ int scenario_stress() {
 
 int scenario_faults() {
     Limits lim;
+    lim.small_options = 16384;
     World w((size_t)1 << 32, lim, false, 32, 12);
     w.ex.fail_launch_every = 7;
     w.ex.fail_batch_every = 5;
@@ -364,11 +372,49 @@ int scenario_memory() {
     return 0;
 }
 
+// Small requests complete beside a slow bulk batch, not behind it: one huge
+// request is launched (its batch takes ~200 ms), then small requests are
+// submitted; each must finish while the huge one is still running.
+int scenario_lanes() {
+    Limits lim;
+    lim.small_options = 65536;
+    HostExec ex;
+    ex.arena = ((size_t)1 << 34);
+    ex.small_options = lim.small_options;
+    ex.ns_per_option = 0.0;
+    ex.fixed_ns = 0;
+    Loop<HostExec> loop(&ex, (size_t)1 << 34, lim, false);
+    int64_t big, small;
+    size_t off;
+    CHECK(loop.register_book(int64_t(1) << 20, &big, &off) == kOk);
+    ex.books[off] = book_bytes(int64_t(1) << 20);
+    CHECK(loop.register_book(1000, &small, &off) == kOk);
+    ex.books[off] = book_bytes(1000);
+    ex.ns_per_option = 200.0;   // the 2^20-option batch: ~210 ms; a 1000-option one: 0.2 ms
+    loop.start();
+    int64_t tb;
+    CHECK(loop.submit(big, kGreeks, &tb) == kOk);
+    std::this_thread::sleep_for(std::chrono::milliseconds(5));   // the big batch is in flight
+    std::string msg;
+    for (int i = 0; i < 20; ++i) {
+        int64_t ts;
+        CHECK(loop.submit(small, i % 2 ? kGreeks : kPrice, &ts) == kOk);
+        CHECK(loop.wait(ts, 10 * 1000 * 1000, &msg) == kOk);
+        int st = -1;
+        CHECK(loop.poll(tb, &st) == kOk && st == kRunning);   // the big one is still running
+        CHECK(loop.release(ts) == kOk);
+    }
+    CHECK(loop.wait(tb, 10 * 1000 * 1000, &msg) == kOk);
+    CHECK(loop.release(tb) == kOk);
+    printf("lanes: 20 small requests finished while a bulk batch ran\n");
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc != 2) {
-        fprintf(stderr, "usage: %s stress|faults|shutdown|books|memory\n", argv[0]);
+        fprintf(stderr, "usage: %s stress|faults|shutdown|books|memory|lanes\n", argv[0]);
         return 2;
     }
     const std::string s = argv[1];
@@ -377,6 +423,7 @@ int main(int argc, char** argv) {
     else if (s == "shutdown") scenario_shutdown();
     else if (s == "books") scenario_books();
     else if (s == "memory") scenario_memory();
+    else if (s == "lanes") scenario_lanes();
     else return 2;
     return g_fail.load() ? 1 : 0;
 }

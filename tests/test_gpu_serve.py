@@ -22,10 +22,10 @@ i64 = ctypes.c_int64
 
 
 class Ctx:
-    def __init__(self, capi, budget, use_graph=1, max_req=0, grid=0, cq=0):
+    def __init__(self, capi, budget, use_graph=1, max_req=0, grid=0, cq=0, small=0):
         self.capi = capi
         self.L = capi.load()
-        o = capi.ServeOpts(max_req, 0, use_graph, grid, cq, 0)
+        o = capi.ServeOpts(max_req, 0, use_graph, grid, cq, small)
         h = ctypes.c_void_p()
         capi.call("bs_ctx_create", 0, budget, ctypes.byref(o), ctypes.byref(h))
         self.h = h.value
@@ -76,12 +76,15 @@ def same_bits(a, b):
     return all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("use_graph", [1, 0])
-def test_alone_and_in_the_mix(gpu, use_graph):
+@pytest.mark.parametrize("use_graph,small", [(1, 0), (0, 0), (0, -1), (1, 1024)])
+def test_alone_and_in_the_mix(gpu, use_graph, small):
+    """Each result bit-identical to the batch map, alone and in a 200-request
+    mix; small = 0 puts books of <= 64K options in the small lane, 1024 a
+    lower threshold, -1 one lane for all."""
     rng = np.random.default_rng(7)
     sizes = [1, 3, 4, 5, 1023, 1024, 1025, 4096, 70_001, 262_144]
     bks = [books.make_book(n, books.SEED_BASE + 40 + i) for i, n in enumerate(sizes)]
-    ctx = Ctx(gpu, 512 << 20, use_graph=use_graph)
+    ctx = Ctx(gpu, 512 << 20, use_graph=use_graph, small=small)
     try:
         ids = [ctx.register(b) for b in bks]
         ref = {(i, k): standalone(gpu, bks[i], k) for i in range(len(bks)) for k in (1, 2)}
@@ -100,7 +103,8 @@ def test_alone_and_in_the_mix(gpu, use_graph):
             assert same_bits(ctx.result(t, bks[i].n, k), ref[(i, k)]), (i, k)
             gpu.call("bs_result_release", ctx.h, t)
         st = ctx.stats()
-        print(f"\nuse_graph={use_graph}: launches {st.launches} for {st.requests_done} requests")
+        print(f"\nuse_graph={use_graph} small={small}: launches {st.launches} for "
+              f"{st.requests_done} requests")
         assert st.requests_done == 2 * len(bks) + 200 and st.requests_failed == 0
         assert st.launches < st.requests_done        # requests were fused
     finally:

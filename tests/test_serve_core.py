@@ -37,6 +37,9 @@ def L():
         "sc_enable_completions": ([vp, ctypes.c_int], None),
         "sc_admit_cap": ([vp, i64, i64, P(i64), ctypes.c_int], ctypes.c_int),
         "sc_queued_options": ([vp], i64),
+        "sc_create_lanes": ([i64, i64], vp),
+        "sc_admit_lane": ([vp, i64, ctypes.c_int, P(i64), ctypes.c_int], ctypes.c_int),
+        "sc_queued_lane": ([vp, ctypes.c_int], i64),
         "sc_reap": ([vp, P(i64), P(ctypes.c_int), P(ctypes.c_int), P(i64), i64], i64),
         "sc_admit": ([vp, i64, P(i64), P(i64), ctypes.c_int], ctypes.c_int),
         "sc_complete": ([vp, P(i64), ctypes.c_int, ctypes.c_int, i64], None),
@@ -383,3 +386,65 @@ def test_arena_size_classes_and_reuse(L):
         assert L.sa_free(a, off) == 1
     assert L.sa_largest_free(a) == cap
     L.sa_destroy(a)
+
+
+def test_small_requests_have_their_own_lane(L):
+    """With small_options = 1000: requests of <= 1000 options queue in lane 0
+    and are admitted only into lane-0 batches, FIFO within the lane; a bulk
+    request waiting for memory does not hold them back; cancel and shutdown
+    reach both lanes."""
+    book_n = {}
+    # arena: books (small 1000, big 100000) plus outputs of one big Greeks request
+    cap = 256 * 1024 * 1024
+    h = L.sc_create_lanes(cap, 1000)
+    try:
+        ids = {}
+        for n in (10, 1000, 1001, 100_000):
+            b, off = i64(), i64()
+            assert L.sc_register(h, n, ctypes.byref(b), ctypes.byref(off)) == OK
+            ids[n] = b.value
+            book_n[b.value] = n
+        t = i64()
+        order = []
+        for n in (100_000, 10, 1001, 1000, 10, 100_000):
+            assert L.sc_submit(h, ids[n], PRICE, 0, ctypes.byref(t)) == OK
+            order.append((n, t.value))
+        assert L.sc_queued_lane(h, 0) == 3 and L.sc_queued_lane(h, 1) == 3
+        tk = (i64 * 16)()
+        k = L.sc_admit_lane(h, 1, 0, tk, 16)
+        assert list(tk[:k]) == [tv for n, tv in order if n <= 1000]       # FIFO in lane 0
+        k = L.sc_admit_lane(h, 2, 1, tk, 16)
+        assert list(tk[:k]) == [tv for n, tv in order if n > 1000]        # the bulk lane's own
+        assert L.sc_queued_lane(h, 0) == 0 and L.sc_queued_lane(h, 1) == 0
+        # cancel in the small lane, shutdown with both lanes queued
+        assert L.sc_submit(h, ids[10], PRICE, 0, ctypes.byref(t)) == OK
+        small = t.value
+        assert L.sc_cancel(h, small, 0) == OK and L.sc_queued_lane(h, 0) == 0
+        for n in (10, 100_000):
+            assert L.sc_submit(h, ids[n], GREEKS, 0, ctypes.byref(t)) == OK
+        L.sc_fail_queued(h, SHUTDOWN, 5)
+        assert L.sc_queued_lane(h, 0) == 0 and L.sc_queued_lane(h, 1) == 0
+    finally:
+        L.sc_destroy(h)
+
+
+def test_bulk_head_waiting_for_memory_does_not_block_the_small_lane(L):
+    # the arena holds the books and one big Greeks output; a second big request waits
+    small_n, big_n = 100, 200_000
+    cap = 5 * (4 * big_n + 256) + 20 * 1024 + 5 * 4 * big_n * 2 - 4096
+    h = L.sc_create_lanes(cap, 1000)
+    try:
+        b, off, t = i64(), i64(), i64()
+        assert L.sc_register(h, big_n, ctypes.byref(b), ctypes.byref(off)) == OK
+        big = b.value
+        assert L.sc_register(h, small_n, ctypes.byref(b), ctypes.byref(off)) == OK
+        sm = b.value
+        tk = (i64 * 16)()
+        for _ in range(2):
+            assert L.sc_submit(h, big, GREEKS, 0, ctypes.byref(t)) == OK
+        assert L.sc_admit_lane(h, 1, 1, tk, 16) == 1      # the second big one does not fit
+        assert L.sc_submit(h, sm, PRICE, 0, ctypes.byref(t)) == OK
+        assert L.sc_admit_lane(h, 2, 1, tk, 16) == 0      # bulk head still waits
+        assert L.sc_admit_lane(h, 3, 0, tk, 16) == 1 and tk[0] == t.value   # small lane goes on
+    finally:
+        L.sc_destroy(h)

@@ -8,7 +8,9 @@ nor another batch in flight, nor a resident book; injected launch and batch
 failures reach exactly the requests of that batch and free their outputs;
 shutdown finishes the batches in flight and fails the rest; admission waits
 for released results when the arena is full; the arena returns to the books'
-bytes when every result is released.  The same scenarios run again under
+bytes when every result is released; with lanes on, every batch holds one
+lane's requests in that lane's slots, and small requests finish while a slow
+bulk batch is still running.  The same scenarios run again under
 ThreadSanitizer."""
 import os
 import shutil
@@ -18,7 +20,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NATIVE = os.path.join(HERE, "native")
-SCENARIOS = ["stress", "faults", "shutdown", "books", "memory"]
+SCENARIOS = ["stress", "faults", "shutdown", "books", "memory", "lanes"]
 
 
 def build(target):
