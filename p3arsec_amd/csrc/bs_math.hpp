@@ -28,8 +28,8 @@
 //            v_rcp_f32, v_sqrt_f32), Horner polynomial with FMA.  About a
 //            third of Ref32's VALU work; prices differ from the reference by
 //            up to ~1.2e-4 on a 10M book (mean 6e-6).
-//   Ref64    fptype = double: the same statements in FP64, no FMA contraction
-//            (its log and CNDF reciprocal are written out below).
+//   Ref64    fptype = double: the same statements in FP64 (its log, CNDF
+//            reciprocal and CNDF polynomial are written out below).
 //
 // Every function here is compiled with FMA contraction off; each FMA is
 // written out, so every kernel that inlines these functions computes the same
@@ -263,13 +263,14 @@ __device__ __forceinline__ double log64(double x) {
     return x == 0.0 ? -__builtin_inf() : (x > 0.0 ? x : __builtin_nan(""));
 }
 
-// Ref64: fptype = double, the reference's statement order with every step in
-// FP64 and no FMA contraction.  Division and square root are IEEE (the
-// compiler's correctly rounded sequences) and exp is the device library's;
-// log is log64 and CNDF's reciprocal 1 / (1 + p|x|) is recip64, its argument
-// capped at 1e10 (past |x| = 38.6 the result is already exactly 0 or 1, and
-// the cap keeps the denominator finite).  Against the reference compiled with
-// fptype = double, prices differ by a few units in the last place.
+// Ref64: fptype = double, the reference's statements with every step in FP64.
+// Division and square root are IEEE (the compiler's correctly rounded
+// sequences) and exp is the device library's; log is log64; CNDF's
+// reciprocal 1 / (1 + p|x|) is recip64, its argument capped at 1e10 (past
+// |x| = 38.6 the result is already exactly 0 or 1, and the cap keeps the
+// denominator finite), and CNDF's polynomial is evaluated by Horner's rule
+// with FMA rather than from the powers of k.  Against the reference compiled
+// with fptype = double, prices differ by a few units in the last place.
 struct Ref64 {
     using T = double;
     __device__ static __forceinline__ double exp(double x) { return ::exp(x); }
@@ -285,14 +286,14 @@ struct Ref64 {
         double ax = neg ? -x : x;
         ax = ax > 1e10 ? 1e10 : ax;
         const double n = exp(-0.5 * ax * ax) * kInvSqrt2Pi;
-        const double k = recip64(1.0 + kP * ax);
-        const double k2 = k * k, k3 = k2 * k, k4 = k3 * k, k5 = k4 * k;
-        double acc = k2 * kB2;
-        acc = acc + k3 * kB3;
-        acc = acc + k4 * kB4;
-        acc = acc + k5 * kB5;
-        const double t1 = acc + k * kB1;
-        const double y = 1.0 - t1 * n;
+        const double k = recip64(__builtin_fma(kP, ax, 1.0));
+        // the A&S polynomial by Horner's rule, one FMA per coefficient
+        double t1 = __builtin_fma(k, kB5, kB4);
+        t1 = __builtin_fma(k, t1, kB3);
+        t1 = __builtin_fma(k, t1, kB2);
+        t1 = __builtin_fma(k, t1, kB1);
+        t1 = t1 * k;
+        const double y = __builtin_fma(-t1, n, 1.0);
         return neg ? 1.0 - y : y;
     }
 };
