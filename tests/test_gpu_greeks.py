@@ -104,3 +104,24 @@ def test_greeks_ragged_and_misaligned(gpu, n):
     outs = gpu.greeks(db)
     mis = np.stack([o.download(np.float32, n) for o in outs]).astype(np.float64)
     assert np.array_equal(base, mis)
+
+
+def test_price_output_on_degenerate_inputs(gpu):
+    """On the degenerate book (T = 0, sigma = 0, zero and infinite S and K,
+    NaN), the fused kernel's price output equals bs_price_f32's bit for bit,
+    NaNs in the same places: the forward pass is the same code."""
+    book = books.degenerate_book()
+    db = gpu.DeviceBook(book)
+    want = gpu.price(db).download(np.float32, book.n)
+    got = [o.download(np.float32, book.n) for o in gpu.greeks(db)]
+    assert np.array_equal(got[0].view(np.uint32), want.view(np.uint32))
+    # the Greeks are finite for the rows whose inputs are ordinary (the rest meet
+    # overflow or 0 * inf in float, as any float adjoint would)
+    def within(x, lo, hi):
+        return (x >= lo) & (x <= hi)
+    ordinary = within(book.s, 1e-3, 1e6) & within(book.k, 1e-3, 1e6) & within(book.v, 1e-3, 10) \
+        & within(book.t, 1e-3, 100) & within(np.abs(book.r), 0, 1)
+    assert ordinary.sum() >= 4
+    for g in got[1:]:
+        assert np.all(np.isfinite(g[ordinary]))
+    db.free()
