@@ -242,7 +242,7 @@ const BookRec* Core::book(int64_t id) const {
     return it == books_.end() ? nullptr : &it->second;
 }
 
-int Core::check_submit(int64_t book, int kind) {
+int Core::check_submit(int64_t book, int kind, BookRec** rec) {
     if (kind != kPrice && kind != kGreeks) return kErrInvalid;
     auto it = books_.find(book);
     if (it == books_.end()) return kErrNotFound;
@@ -250,26 +250,32 @@ int Core::check_submit(int64_t book, int kind) {
     // refused now instead of waiting for memory that will never come.
     if (output_bytes(it->second.n, kind) + book_bytes(it->second.n) > arena_.capacity())
         return kErrBudget;
+    *rec = &it->second;
     return kOk;
 }
 
-int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket, int64_t user) {
-    const int rc = check_submit(book, kind);
-    if (rc != kOk) return rc;
-    BookRec* b = &books_.find(book)->second;
-    Request* r;
-    if (new_ticket(&r) == 0) return kErrNoMem;
-    r->book = book;
+// Queues a request on a checked book (b from check_submit) with a ticket from
+// new_ticket.
+void Core::enqueue(BookRec* b, int kind, int64_t now, int64_t user, Request* r) {
     r->bookp = b;
     r->user = user;
-    r->kind = kind;
+    r->kind = (int8_t)kind;
     r->n = b->n;
-    r->lane = lane_of(r->n);
+    r->lane = (int8_t)lane_of(r->n);
     r->t_submit = now;
     pending_[r->lane].push_back(r->ticket);
     pending_options_[r->lane] += r->n;
     ++b->refs;
     ++st_.requests_queued;
+}
+
+int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket, int64_t user) {
+    BookRec* b;
+    const int rc = check_submit(book, kind, &b);
+    if (rc != kOk) return rc;
+    Request* r;
+    if (new_ticket(&r) == 0) return kErrNoMem;
+    enqueue(b, kind, now, user, r);
     *ticket = r->ticket;
     return kOk;
 }
@@ -277,14 +283,21 @@ int Core::submit(int64_t book, int kind, int64_t now, int64_t* ticket, int64_t u
 int Core::submit_many(const int64_t* books, const int* kinds, const int64_t* users,
                       int64_t count, int64_t now, int64_t* tickets) {
     if (count < 0 || (count > 0 && (!books || !kinds || !tickets))) return kErrInvalid;
+    // Book records live in a node-based map, so the pointers stay valid while
+    // this call (which holds the owner's lock) runs.
+    scratch_books_.resize((size_t)count);
     for (int64_t i = 0; i < count; ++i) {
-        const int rc = check_submit(books[i], kinds[i]);
+        const int rc = check_submit(books[i], kinds[i], &scratch_books_[(size_t)i]);
         if (rc != kOk) return rc;
     }
     if (nslots_ - (int64_t)spare_slots_.size() + count > (int64_t(1) << kSlotBits))
         return kErrNoMem;
-    for (int64_t i = 0; i < count; ++i)
-        submit(books[i], kinds[i], now, &tickets[i], users ? users[i] : 0);
+    for (int64_t i = 0; i < count; ++i) {
+        Request* r;
+        new_ticket(&r);   // cannot fail: the slots were counted above
+        enqueue(scratch_books_[(size_t)i], kinds[i], now, users ? users[i] : 0, r);
+        tickets[i] = r->ticket;
+    }
     return kOk;
 }
 
@@ -324,9 +337,8 @@ size_t Core::admit(int64_t now, std::vector<Admitted>* out, int64_t option_cap, 
         pending.pop_front();
         pending_options_[lane] -= r->n;
         r->out_off = o;
-        r->out_blk = h;
+        r->out_blk = (int32_t)h;
         r->state = kRunning;
-        r->t_launch = now;
         st_.queue_ns += now - r->t_submit;
         options += r->n;
         out->push_back({r->ticket, r->n, r->bookp->off, o, r->bookp->n, r->kind});
@@ -368,7 +380,7 @@ void Core::fail_queued(int status, int64_t now) {
             pending_[lane].pop_front();
             pending_options_[lane] -= r->n;
             r->state = kFailed;
-            r->status = status;
+            r->status = (int8_t)status;
             r->t_done = now;
             r->bookp->refs--;
             --st_.requests_queued;

@@ -91,20 +91,23 @@ struct BookRec {
     int refs = 0;  // queued + running requests
 };
 
-struct Request {
-    int64_t ticket = 0;  // 0: slot unused
-    int64_t book = 0;
+// One cache line: a request is written by the submitting thread, the
+// dispatcher and the releasing thread in turn, so each hand-over moves one
+// line between cores.
+struct alignas(64) Request {
+    int64_t ticket = 0;        // 0: slot unused
     BookRec* bookp = nullptr;  // stable: a referenced book cannot be released
     int64_t user = 0;
-    int kind = kPrice;
     int64_t n = 0;
-    int lane = kLaneBulk;
-    int state = kQueued;
-    int status = kOk;
+    int64_t t_submit = 0, t_done = 0;
     size_t out_off = 0;
-    int64_t out_blk = -1;
-    int64_t t_submit = 0, t_launch = 0, t_done = 0;
+    int32_t out_blk = -1;      // arena handle of the outputs
+    int8_t kind = kPrice;
+    int8_t lane = kLaneBulk;
+    int8_t state = kQueued;
+    int8_t status = kOk;       // kOk or a (small, negative) error code
 };
+static_assert(sizeof(Request) == 64, "Request is one cache line");
 
 // What the executor needs of an admitted request to build its descriptor;
 // a copy, so the table can be written after the lock is released.
@@ -202,7 +205,8 @@ class Core {
     Request* slot_of(int64_t ticket);
     int64_t new_ticket(Request** r);
     void free_slot(Request* r);
-    int check_submit(int64_t book, int kind);
+    int check_submit(int64_t book, int kind, BookRec** rec);
+    void enqueue(BookRec* b, int kind, int64_t now, int64_t user, Request* r);
     void finished(const Request& r) {
         if (cq_) completed_.push_back(r.ticket);
     }
@@ -219,6 +223,7 @@ class Core {
     int64_t pending_options_[kLanes] = {0, 0};
     bool cq_ = false;
     Stats st_;
+    std::vector<BookRec*> scratch_books_;   // submit_many's checked books
 };
 
 }  // namespace serve
