@@ -421,6 +421,21 @@ def main() -> int:
 
     if not args.no_extra:
         try:
+            # the whole timed book against the reference (the bit-exact FP32
+            # port) and against two FP64 oracles: the reference's algorithm in
+            # double, and the exact closed form
+            log("bench: accuracy of the 10M book")
+            ref_full, _, _ = pyoracle.price_f32(host)
+            g64 = got.astype(np.float64)
+            acc = {}
+            for name, want in (("reference_fp32", ref_full),
+                               ("fp64_restatement", pyoracle.price_f64(host.astype(np.float64))),
+                               ("fp64_closed_form", pyoracle.price_exact(host))):
+                d = np.abs(g64 - want)
+                acc[name] = {"max_abs_err": float(d.max()), "mean_abs_err": float(d.mean())}
+            acc["bit_identical_to_reference"] = float(np.mean(got == ref_full))
+            result["accuracy_10m"] = acc
+
             def launch_fast(i, s):
                 capi.check(L.bs_price_fast_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s), "fast")
             log("bench: fast price map")

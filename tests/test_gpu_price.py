@@ -142,6 +142,11 @@ def test_full_size_10m_against_oracle_and_parity(gpu, oracle):
     mx, mean = err_stats(got, ref)
     print(f"\n10M vs oracle: max {mx:.3e} mean {mean:.3e} "
           f"bit-identical {np.mean(got == ref):.5f}")
+    ex = err_stats(got, oracle.price_exact(book))
+    r64 = err_stats(got, oracle.price_f64(book.astype(np.float64)))
+    print(f"10M vs FP64 closed form: max {ex[0]:.3e} mean {ex[1]:.3e}; "
+          f"vs FP64 restatement: max {r64[0]:.3e} mean {r64[1]:.3e}")
+    assert ex[0] <= 2e-4 and r64[0] <= 2e-4
     fast = run(gpu, book, "fast")
     fmx, fmean = err_stats(fast, ref)
     print(f"10M fast vs oracle: max {fmx:.3e} mean {fmean:.3e} "
