@@ -1,5 +1,6 @@
 // bs_internal.hpp -- shared host-side declarations of libbs_mi355.so.
 #pragma once
+#include <atomic>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -11,13 +12,15 @@ namespace bs {
 
 constexpr int kThreads = 256;  // threads per block: 4 wave64s
 
+// Process-wide launch settings (bs_config_set).  Atomics: any thread may set
+// them while others launch; a launch reads each setting once.
 struct Config {
-    int check_pointers = 1;
-    int64_t max_blocks = 0;  // 0: one 4-option group per thread, no grid cap
-    int vector_width = 0;  // 0: each kernel's measured default (2 or 4)
-    int nontemporal = 1;
-    int pipeline = -1;     // -1: each kernel's measured default
-    int block_threads = 0;  // streaming kernels' block size: 0 = each kernel's default
+    std::atomic<int> check_pointers{1};
+    std::atomic<int64_t> max_blocks{0};  // 0: one group per work-item, no grid cap
+    std::atomic<int> vector_width{0};    // 0: each kernel's measured default (2 or 4)
+    std::atomic<int> nontemporal{1};
+    std::atomic<int> pipeline{-1};       // -1: each kernel's measured default
+    std::atomic<int> block_threads{0};   // streaming kernels' block size: 0 = each kernel's default
 };
 Config& config();
 

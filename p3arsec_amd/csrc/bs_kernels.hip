@@ -118,8 +118,10 @@ template <class Op, bool NT, bool PIPE>
 hipError_t launch_form(const Op& op, int64_t n, hipStream_t st) {
     constexpr int VW = Op::kVW;
     const int64_t ngroups = n / VW;
-    const int64_t cap = config().max_blocks > 0 ? config().max_blocks : Op::kBlocks;
-    const int nt = config().block_threads > 0 ? config().block_threads : Op::kThreadsPref;
+    const int64_t mb = config().max_blocks.load(std::memory_order_relaxed);
+    const int bt = config().block_threads.load(std::memory_order_relaxed);
+    const int64_t cap = mb > 0 ? mb : Op::kBlocks;
+    const int nt = bt > 0 ? bt : Op::kThreadsPref;
     const unsigned grid = grid_for(std::max<int64_t>(ngroups, 1), cap, nt);
     hipLaunchKernelGGL((k_stream<Op, NT, PIPE>), dim3(grid), dim3(nt), 0, st, op, ngroups,
                        ngroups * VW, n);
@@ -128,8 +130,9 @@ hipError_t launch_form(const Op& op, int64_t n, hipStream_t st) {
 
 template <class Op>
 hipError_t launch_op(const Op& op, int64_t n, hipStream_t st) {
-    const bool nt = config().nontemporal != 0;
-    const bool pipe = config().pipeline < 0 ? Op::kPipe : config().pipeline != 0;
+    const bool nt = config().nontemporal.load(std::memory_order_relaxed) != 0;
+    const int pl = config().pipeline.load(std::memory_order_relaxed);
+    const bool pipe = pl < 0 ? Op::kPipe : pl != 0;
     if (pipe)
         return nt ? launch_form<Op, true, true>(op, n, st) : launch_form<Op, false, true>(op, n, st);
     return nt ? launch_form<Op, true, false>(op, n, st) : launch_form<Op, false, false>(op, n, st);
@@ -142,7 +145,8 @@ template <template <int> class OP, class Args, class AlignFn>
 hipError_t launch_stream(const Args& a, int64_t n, int preferred_vw, AlignFn aligned_for,
                          hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    int vw = config().vector_width > 0 ? config().vector_width : preferred_vw;
+    const int cvw = config().vector_width.load(std::memory_order_relaxed);
+    int vw = cvw > 0 ? cvw : preferred_vw;
     if (vw != 2 && vw != 4) vw = 4;
     if (vw == 4 && !aligned_for(4)) vw = 2;
     if (!aligned_for(vw)) {
