@@ -22,8 +22,21 @@ Config& config() {
 
 static thread_local std::string t_last_error;
 
-int set_error(int status, const std::string& msg) {
-    t_last_error = msg;
+int set_error(int status, const std::string& msg) noexcept {
+    try {
+        t_last_error = msg;
+    } catch (...) {
+        t_last_error.clear();
+    }
+    return status;
+}
+
+int guard_failed(int status, const char* what, const char* detail) noexcept {
+    try {
+        t_last_error = std::string(what) + ": " + detail;
+    } catch (...) {
+        t_last_error.clear();   // no room even for the message
+    }
     return status;
 }
 
@@ -399,37 +412,39 @@ int bs_repack_aos_f32(const void* aos, int64_t n, float* s, float* k, float* r, 
 int bs_price_passes_f32(const float* s, const float* k, const float* r, const float* v,
                         const float* t, const void* otype, int otype_bytes, float* price,
                         int64_t n, int passes, bs_stream_t stream) {
-    int rc;
-    if ((rc = check_n(n))) return rc;
-    if (passes < 1) return set_error(BS_ERR_INVALID, "passes < 1");
-    if (otype_bytes != 1 && otype_bytes != 4)
-        return set_error(BS_ERR_INVALID, "otype_bytes must be 1 or 4");
-    if (n == 0) return BS_OK;
-    if ((rc = check_soa(s, k, r, v, t, n)) ||
-        (rc = check_device_range(otype, (size_t)n * otype_bytes, "otype")) ||
-        (rc = check_device_range(price, (size_t)n * 4, "price")))
-        return rc;
-    const SoaIn<float> in{s, k, r, v, t};
-    const hipStream_t st = as_stream(stream);
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_error(e, "hipGetDevice");
-    if (n <= kPassGraphMaxOptions && passes > 1 && dev < kPassDevices) {
-        const Config& cf = config();
-        const PassKey key{{s, k, r, v, t, otype, price}, n, passes, otype_bytes, dev,
-                          {cf.max_blocks, cf.vector_width, cf.nontemporal, cf.pipeline,
-                           cf.block_threads}};
-        PassGraphCache& c = pass_cache();
-        std::lock_guard<std::mutex> lk(c.mu);
-        PassGraph* g = pass_graph(c, key, in, otype, price, &e);
-        if (!g) return hip_error(e, "bs_price_passes_f32: graph capture");
-        e = hipGraphLaunch(g->exec, st);
-        if (e == hipSuccess) e = hipEventRecord(g->done, st);
+    return guarded("bs_price_passes_f32", [&]() -> int {
+        int rc;
+        if ((rc = check_n(n))) return rc;
+        if (passes < 1) return set_error(BS_ERR_INVALID, "passes < 1");
+        if (otype_bytes != 1 && otype_bytes != 4)
+            return set_error(BS_ERR_INVALID, "otype_bytes must be 1 or 4");
+        if (n == 0) return BS_OK;
+        if ((rc = check_soa(s, k, r, v, t, n)) ||
+            (rc = check_device_range(otype, (size_t)n * otype_bytes, "otype")) ||
+            (rc = check_device_range(price, (size_t)n * 4, "price")))
+            return rc;
+        const SoaIn<float> in{s, k, r, v, t};
+        const hipStream_t st = as_stream(stream);
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+        if (n <= kPassGraphMaxOptions && passes > 1 && dev < kPassDevices) {
+            const Config& cf = config();
+            const PassKey key{{s, k, r, v, t, otype, price}, n, passes, otype_bytes, dev,
+                              {cf.max_blocks, cf.vector_width, cf.nontemporal, cf.pipeline,
+                               cf.block_threads}};
+            PassGraphCache& c = pass_cache();
+            std::lock_guard<std::mutex> lk(c.mu);
+            PassGraph* g = pass_graph(c, key, in, otype, price, &e);
+            if (!g) return hip_error(e, "bs_price_passes_f32: graph capture");
+            e = hipGraphLaunch(g->exec, st);
+            if (e == hipSuccess) e = hipEventRecord(g->done, st);
+            return done(e, "bs_price_passes_f32");
+        }
+        for (int p = 0; p < passes && e == hipSuccess; ++p)
+            e = launch_pass(in, otype, otype_bytes, price, n, st);
         return done(e, "bs_price_passes_f32");
-    }
-    for (int p = 0; p < passes && e == hipSuccess; ++p)
-        e = launch_pass(in, otype, otype_bytes, price, n, st);
-    return done(e, "bs_price_passes_f32");
+    });
 }
 
 int bs_price_host_f32(const float* s, const float* k, const float* r, const float* v,
@@ -475,50 +490,54 @@ static struct {
 
 int bs_bind_globals(int num_options, int nthreads, float* sptprice, float* strike, float* rate,
                     float* volatility, float* otime, int* otype, float* prices) {
-    if (num_options < 0 || nthreads < 1)
-        return set_error(BS_ERR_INVALID, "num_options < 0 or nthreads < 1");
-    std::lock_guard<std::mutex> lk(g_globals.mu);
-    g_globals.num_options = num_options;
-    g_globals.nthreads = nthreads;
-    g_globals.sptprice = sptprice;
-    g_globals.strike = strike;
-    g_globals.rate = rate;
-    g_globals.volatility = volatility;
-    g_globals.otime = otime;
-    g_globals.otype = otype;
-    g_globals.prices = prices;
-    return BS_OK;
+    return guarded("bs_bind_globals", [&]() -> int {
+        if (num_options < 0 || nthreads < 1)
+            return set_error(BS_ERR_INVALID, "num_options < 0 or nthreads < 1");
+        std::lock_guard<std::mutex> lk(g_globals.mu);
+        g_globals.num_options = num_options;
+        g_globals.nthreads = nthreads;
+        g_globals.sptprice = sptprice;
+        g_globals.strike = strike;
+        g_globals.rate = rate;
+        g_globals.volatility = volatility;
+        g_globals.otime = otime;
+        g_globals.otype = otype;
+        g_globals.prices = prices;
+        return BS_OK;
+    });
 }
 
 int bs_thread_compat(void* tid_ptr) {
-    constexpr int kNumRuns = 100;  // NUM_RUNS, blackscholes.c:87
-    if (!tid_ptr) return set_error(BS_ERR_INVALID, "tid_ptr is NULL");
-    const int tid = *static_cast<int*>(tid_ptr);
-    int n, nt;
-    float *s, *k, *r, *v, *t, *out;
-    int* ty;
-    {
-        std::lock_guard<std::mutex> lk(g_globals.mu);
-        n = g_globals.num_options;
-        nt = g_globals.nthreads;
-        s = g_globals.sptprice;
-        k = g_globals.strike;
-        r = g_globals.rate;
-        v = g_globals.volatility;
-        t = g_globals.otime;
-        ty = g_globals.otype;
-        out = g_globals.prices;
-    }
-    if (tid < 0 || tid >= nt) return set_error(BS_ERR_INVALID, "tid outside [0, nthreads)");
-    // Block of the static partition (blackscholes.c:613-614): N/nthreads
-    // options from tid*(N/nthreads); with one thread that is the whole book.
-    const int64_t len = n / nt;
-    const int64_t start = (int64_t)tid * len;
-    if (len == 0) return BS_OK;
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    return bs_price_host_f32(s + start, k + start, r + start, v + start, t + start, ty + start,
-                             out + start, len, kNumRuns, dev);
+    return guarded("bs_thread_compat", [&]() -> int {
+        constexpr int kNumRuns = 100;  // NUM_RUNS, blackscholes.c:87
+        if (!tid_ptr) return set_error(BS_ERR_INVALID, "tid_ptr is NULL");
+        const int tid = *static_cast<int*>(tid_ptr);
+        int n, nt;
+        float *s, *k, *r, *v, *t, *out;
+        int* ty;
+        {
+            std::lock_guard<std::mutex> lk(g_globals.mu);
+            n = g_globals.num_options;
+            nt = g_globals.nthreads;
+            s = g_globals.sptprice;
+            k = g_globals.strike;
+            r = g_globals.rate;
+            v = g_globals.volatility;
+            t = g_globals.otime;
+            ty = g_globals.otype;
+            out = g_globals.prices;
+        }
+        if (tid < 0 || tid >= nt) return set_error(BS_ERR_INVALID, "tid outside [0, nthreads)");
+        // Block of the static partition (blackscholes.c:613-614): N/nthreads
+        // options from tid*(N/nthreads); with one thread that is the whole book.
+        const int64_t len = n / nt;
+        const int64_t start = (int64_t)tid * len;
+        if (len == 0) return BS_OK;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        return bs_price_host_f32(s + start, k + start, r + start, v + start, t + start, ty + start,
+                                 out + start, len, kNumRuns, dev);
+    });
 }
 
 // ---- device plumbing -----------------------------------------------------------------

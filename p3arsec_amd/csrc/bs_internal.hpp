@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <new>
+#include <stdexcept>
 #include <string>
 
 #include "../../include/bs_mi355.h"
@@ -24,8 +26,24 @@ struct Config {
 };
 Config& config();
 
+// Runs an entry point's body so that no C++ exception leaves the C ABI: an
+// allocation failure becomes BS_ERR_NOMEM, anything else BS_ERR_HIP.
+int guard_failed(int status, const char* what, const char* detail) noexcept;
+template <class F>
+int guarded(const char* what, F&& body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return guard_failed(BS_ERR_NOMEM, what, "out of host memory");
+    } catch (const std::exception& e) {
+        return guard_failed(BS_ERR_HIP, what, e.what());
+    } catch (...) {
+        return guard_failed(BS_ERR_HIP, what, "unknown exception");
+    }
+}
+
 // Thread-local error text behind bs_last_error().
-int set_error(int status, const std::string& msg);
+int set_error(int status, const std::string& msg) noexcept;
 int hip_error(hipError_t e, const char* what);
 
 // Validates that [p, p + bytes) is memory the GPU may access.

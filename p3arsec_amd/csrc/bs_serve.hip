@@ -364,233 +364,272 @@ int status(int rc, const char* what) { return rc == BS_OK ? BS_OK : set_error(rc
 extern "C" {
 
 int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts, bs_ctx_t* out) {
-    if (!out) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    *out = nullptr;
-    if (hbm_budget_bytes < serve::kAlign * 16)
-        return set_error(BS_ERR_INVALID, "hbm_budget_bytes too small");
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || device < 0 || device >= ndev)
-        return set_error(BS_ERR_NODEVICE, "device index out of range");
-    serve::Limits lim;
-    lim.max_batch_requests = kMaxDesc;
-    lim.max_batch_options = int64_t(1) << 31;
-    lim.small_options = 65536;
-    int use_graph = 0, grid_blocks = 0;
-    if (opts) {
-        if (opts->small_request_options != 0)
-            lim.small_options = opts->small_request_options < 0 ? 0 : opts->small_request_options;
-        if (opts->max_batch_requests > 0)
-            lim.max_batch_requests = std::min<int64_t>(opts->max_batch_requests, kMaxDesc);
-        if (opts->max_batch_options > 0) lim.max_batch_options = opts->max_batch_options;
-        use_graph = opts->use_graph;
-        grid_blocks = opts->grid_blocks;
-    }
-    auto* c = new bs_ctx_s();
-    e = c->exec.init(device, hbm_budget_bytes, use_graph, grid_blocks);
-    if (e != hipSuccess) {
-        const int rc = hip_error(e, "bs_ctx_create");
-        c->exec.release();
-        delete c;
-        return rc;
-    }
-    c->loop = new serve::Loop<HipExec>(&c->exec, hbm_budget_bytes, lim,
-                                       opts && opts->completion_queue != 0);
-    c->loop->start();
-    *out = c;
-    return BS_OK;
+    return guarded("bs_ctx_create", [&]() -> int {
+        if (!out) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        *out = nullptr;
+        if (hbm_budget_bytes < serve::kAlign * 16)
+            return set_error(BS_ERR_INVALID, "hbm_budget_bytes too small");
+        int ndev = 0;
+        hipError_t e = hipGetDeviceCount(&ndev);
+        if (e != hipSuccess || device < 0 || device >= ndev)
+            return set_error(BS_ERR_NODEVICE, "device index out of range");
+        serve::Limits lim;
+        lim.max_batch_requests = kMaxDesc;
+        lim.max_batch_options = int64_t(1) << 31;
+        lim.small_options = 65536;
+        int use_graph = 0, grid_blocks = 0;
+        if (opts) {
+            if (opts->small_request_options != 0)
+                lim.small_options = opts->small_request_options < 0 ? 0 : opts->small_request_options;
+            if (opts->max_batch_requests > 0)
+                lim.max_batch_requests = std::min<int64_t>(opts->max_batch_requests, kMaxDesc);
+            if (opts->max_batch_options > 0) lim.max_batch_options = opts->max_batch_options;
+            use_graph = opts->use_graph;
+            grid_blocks = opts->grid_blocks;
+        }
+        auto* c = new bs_ctx_s();
+        e = c->exec.init(device, hbm_budget_bytes, use_graph, grid_blocks);
+        if (e != hipSuccess) {
+            const int rc = hip_error(e, "bs_ctx_create");
+            c->exec.release();
+            delete c;
+            return rc;
+        }
+        try {
+            c->loop = new serve::Loop<HipExec>(&c->exec, hbm_budget_bytes, lim,
+                                               opts && opts->completion_queue != 0);
+            c->loop->start();
+        } catch (...) {   // free the device resources, then report the failure
+            delete c->loop;
+            c->exec.release();
+            delete c;
+            throw;
+        }
+        *out = c;
+        return BS_OK;
+    });
 }
 
 int bs_ctx_destroy(bs_ctx_t ctx) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    if (c->loop) c->loop->stop();
-    c->exec.release();
-    delete c->loop;
-    delete c;
-    return BS_OK;
+    return guarded("bs_ctx_destroy", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        if (c->loop) c->loop->stop();
+        c->exec.release();
+        delete c->loop;
+        delete c;
+        return BS_OK;
+    });
 }
 
 int bs_book_register(bs_ctx_t ctx, const float* s, const float* k, const float* r, const float* v,
                      const float* t, const uint8_t* otype, int64_t n, bs_book_t* book) {
-    bs_ctx_s* c = ctx;
-    if (!c || !book) return set_error(BS_ERR_INVALID, "ctx or book is NULL");
-    if (n <= 0) return set_error(BS_ERR_INVALID, "n <= 0");
-    if (!s || !k || !r || !v || !t || !otype) return set_error(BS_ERR_INVALID, "NULL array");
-    int64_t id;
-    size_t off;
-    const int rc = c->loop->register_book(n, &id, &off);
-    if (rc == BS_ERR_SHUTDOWN) return set_error(rc, "context shutting down");
-    if (rc != BS_OK) return set_error(rc, "book does not fit in the arena");
-    size_t bo[6];
-    serve::book_layout(n, bo);
-    const void* src[6] = {s, k, r, v, t, otype};
-    hipError_t e = hipSetDevice(c->exec.device);
-    for (int j = 0; j < 6 && e == hipSuccess; ++j)
-        e = hipMemcpy(c->exec.arena + off + bo[j], src[j], (size_t)n * (j < 5 ? 4 : 1),
-                      hipMemcpyHostToDevice);
-    // The batches run on non-blocking streams: the book must be in device
-    // memory before its id is handed out.
-    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
-    if (e != hipSuccess) {
-        c->loop->release_book(id);
-        return hip_error(e, "bs_book_register");
-    }
-    *book = id;
-    return BS_OK;
+    return guarded("bs_book_register", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c || !book) return set_error(BS_ERR_INVALID, "ctx or book is NULL");
+        if (n <= 0) return set_error(BS_ERR_INVALID, "n <= 0");
+        if (!s || !k || !r || !v || !t || !otype) return set_error(BS_ERR_INVALID, "NULL array");
+        int64_t id;
+        size_t off;
+        const int rc = c->loop->register_book(n, &id, &off);
+        if (rc == BS_ERR_SHUTDOWN) return set_error(rc, "context shutting down");
+        if (rc != BS_OK) return set_error(rc, "book does not fit in the arena");
+        size_t bo[6];
+        serve::book_layout(n, bo);
+        const void* src[6] = {s, k, r, v, t, otype};
+        hipError_t e = hipSetDevice(c->exec.device);
+        for (int j = 0; j < 6 && e == hipSuccess; ++j)
+            e = hipMemcpy(c->exec.arena + off + bo[j], src[j], (size_t)n * (j < 5 ? 4 : 1),
+                          hipMemcpyHostToDevice);
+        // The batches run on non-blocking streams: the book must be in device
+        // memory before its id is handed out.
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) {
+            c->loop->release_book(id);
+            return hip_error(e, "bs_book_register");
+        }
+        *book = id;
+        return BS_OK;
+    });
 }
 
 int bs_book_release(bs_ctx_t ctx, bs_book_t book) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    return status(c->loop->release_book(book), "bs_book_release");
+    return guarded("bs_book_release", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        return status(c->loop->release_book(book), "bs_book_release");
+    });
 }
 
 int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t* ticket) {
-    bs_ctx_s* c = ctx;
-    if (!c || !ticket) return set_error(BS_ERR_INVALID, "ctx or ticket is NULL");
-    return status(c->loop->submit(book, kind, ticket), "bs_submit");
+    return guarded("bs_submit", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c || !ticket) return set_error(BS_ERR_INVALID, "ctx or ticket is NULL");
+        return status(c->loop->submit(book, kind, ticket), "bs_submit");
+    });
 }
 
 int bs_submit_many(bs_ctx_t ctx, const bs_book_t* books, const int* kinds, const int64_t* users,
                    int64_t count, bs_ticket_t* tickets) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    if (count < 0 || (count > 0 && (!books || !kinds || !tickets)))
-        return set_error(BS_ERR_INVALID, "bad request arrays");
-    return status(c->loop->submit_many(books, kinds, users, count, tickets), "bs_submit_many");
+    return guarded("bs_submit_many", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        if (count < 0 || (count > 0 && (!books || !kinds || !tickets)))
+            return set_error(BS_ERR_INVALID, "bad request arrays");
+        return status(c->loop->submit_many(books, kinds, users, count, tickets), "bs_submit_many");
+    });
 }
 
 int bs_reap(bs_ctx_t ctx, bs_completion* out, int64_t max, int64_t timeout_us, int64_t* count) {
-    static_assert(sizeof(bs_completion) == sizeof(serve::Completion) &&
-                      offsetof(bs_completion, user) == offsetof(serve::Completion, user) &&
-                      offsetof(bs_completion, state) == offsetof(serve::Completion, state) &&
-                      offsetof(bs_completion, done_ns) == offsetof(serve::Completion, t_done),
-                  "bs_completion layout");
-    bs_ctx_s* c = ctx;
-    if (!c || !count) return set_error(BS_ERR_INVALID, "ctx or count is NULL");
-    *count = 0;
-    if (max < 0 || (max > 0 && !out)) return set_error(BS_ERR_INVALID, "bad output array");
-    const int rc = c->loop->reap(reinterpret_cast<serve::Completion*>(out), max, timeout_us, count);
-    return rc == BS_OK ? BS_OK : set_error(rc, "context has no completion queue");
+    return guarded("bs_reap", [&]() -> int {
+        static_assert(sizeof(bs_completion) == sizeof(serve::Completion) &&
+                          offsetof(bs_completion, user) == offsetof(serve::Completion, user) &&
+                          offsetof(bs_completion, state) == offsetof(serve::Completion, state) &&
+                          offsetof(bs_completion, done_ns) == offsetof(serve::Completion, t_done),
+                      "bs_completion layout");
+        bs_ctx_s* c = ctx;
+        if (!c || !count) return set_error(BS_ERR_INVALID, "ctx or count is NULL");
+        *count = 0;
+        if (max < 0 || (max > 0 && !out)) return set_error(BS_ERR_INVALID, "bad output array");
+        const int rc = c->loop->reap(reinterpret_cast<serve::Completion*>(out), max, timeout_us, count);
+        return rc == BS_OK ? BS_OK : set_error(rc, "context has no completion queue");
+    });
 }
 
 int bs_result_release_many(bs_ctx_t ctx, const bs_ticket_t* tickets, int64_t count) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    if (count < 0 || (count > 0 && !tickets)) return set_error(BS_ERR_INVALID, "bad ticket array");
-    return status(c->loop->release_many(tickets, count), "bs_result_release_many");
+    return guarded("bs_result_release_many", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        if (count < 0 || (count > 0 && !tickets)) return set_error(BS_ERR_INVALID, "bad ticket array");
+        return status(c->loop->release_many(tickets, count), "bs_result_release_many");
+    });
 }
 
 int bs_poll(bs_ctx_t ctx, bs_ticket_t ticket, int* state) {
-    bs_ctx_s* c = ctx;
-    if (!c || !state) return set_error(BS_ERR_INVALID, "ctx or state is NULL");
-    return status(c->loop->poll(ticket, state), "unknown ticket");
+    return guarded("bs_poll", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c || !state) return set_error(BS_ERR_INVALID, "ctx or state is NULL");
+        return status(c->loop->poll(ticket, state), "unknown ticket");
+    });
 }
 
 int bs_wait(bs_ctx_t ctx, bs_ticket_t ticket, int64_t timeout_us) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    std::string msg;
-    const int rc = c->loop->wait(ticket, timeout_us, &msg);
-    switch (rc) {
-        case BS_OK: return BS_OK;
-        case BS_ERR_TIMEOUT: return set_error(rc, "bs_wait timed out");
-        case BS_ERR_NOT_FOUND: return set_error(rc, "unknown ticket");
-        case BS_ERR_CANCELLED: return set_error(rc, "ticket cancelled");
-        default: return set_error(rc, msg);
-    }
+    return guarded("bs_wait", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        std::string msg;
+        const int rc = c->loop->wait(ticket, timeout_us, &msg);
+        switch (rc) {
+            case BS_OK: return BS_OK;
+            case BS_ERR_TIMEOUT: return set_error(rc, "bs_wait timed out");
+            case BS_ERR_NOT_FOUND: return set_error(rc, "unknown ticket");
+            case BS_ERR_CANCELLED: return set_error(rc, "ticket cancelled");
+            default: return set_error(rc, msg);
+        }
+    });
 }
 
 int bs_result_device(bs_ctx_t ctx, bs_ticket_t ticket, const float** price, const float** delta,
                      const float** vega, const float** rho, const float** theta) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    int64_t n;
-    int kind;
-    size_t off;
-    const int rc = c->loop->result_info(ticket, &n, &kind, &off);
-    if (rc == BS_ERR_NOT_FOUND) return set_error(rc, "unknown ticket");
-    if (rc != BS_OK) return set_error(rc, "ticket not done");
-    size_t oo[5];
-    serve::output_layout(n, kind, oo);
-    const float** dst[5] = {price, delta, vega, rho, theta};
-    for (int j = 0; j < 5; ++j) {
-        if (!dst[j]) continue;
-        *dst[j] = (j == 0 || kind == serve::kGreeks)
-                      ? reinterpret_cast<const float*>(c->exec.arena + off + oo[j])
-                      : nullptr;
-    }
-    return BS_OK;
+    return guarded("bs_result_device", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        int64_t n;
+        int kind;
+        size_t off;
+        const int rc = c->loop->result_info(ticket, &n, &kind, &off);
+        if (rc == BS_ERR_NOT_FOUND) return set_error(rc, "unknown ticket");
+        if (rc != BS_OK) return set_error(rc, "ticket not done");
+        size_t oo[5];
+        serve::output_layout(n, kind, oo);
+        const float** dst[5] = {price, delta, vega, rho, theta};
+        for (int j = 0; j < 5; ++j) {
+            if (!dst[j]) continue;
+            *dst[j] = (j == 0 || kind == serve::kGreeks)
+                          ? reinterpret_cast<const float*>(c->exec.arena + off + oo[j])
+                          : nullptr;
+        }
+        return BS_OK;
+    });
 }
 
 int bs_result_copy(bs_ctx_t ctx, bs_ticket_t ticket, float* price, float* delta, float* vega,
                    float* rho, float* theta) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    int64_t n;
-    int kind;
-    size_t off;
-    const int rc = c->loop->result_info(ticket, &n, &kind, &off);
-    if (rc == BS_ERR_NOT_FOUND) return set_error(rc, "unknown ticket");
-    if (rc != BS_OK) return set_error(rc, "ticket not done");
-    size_t oo[5];
-    serve::output_layout(n, kind, oo);
-    float* dst[5] = {price, delta, vega, rho, theta};
-    hipError_t e = hipSetDevice(c->exec.device);
-    for (int j = 0; j < 5 && e == hipSuccess; ++j) {
-        if (!dst[j]) continue;
-        if (j > 0 && kind != serve::kGreeks) return set_error(BS_ERR_INVALID, "not a Greeks ticket");
-        e = hipMemcpy(dst[j], c->exec.arena + off + oo[j], (size_t)n * 4, hipMemcpyDeviceToHost);
-    }
-    return e == hipSuccess ? BS_OK : hip_error(e, "bs_result_copy");
+    return guarded("bs_result_copy", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        int64_t n;
+        int kind;
+        size_t off;
+        const int rc = c->loop->result_info(ticket, &n, &kind, &off);
+        if (rc == BS_ERR_NOT_FOUND) return set_error(rc, "unknown ticket");
+        if (rc != BS_OK) return set_error(rc, "ticket not done");
+        size_t oo[5];
+        serve::output_layout(n, kind, oo);
+        float* dst[5] = {price, delta, vega, rho, theta};
+        hipError_t e = hipSetDevice(c->exec.device);
+        for (int j = 0; j < 5 && e == hipSuccess; ++j) {
+            if (!dst[j]) continue;
+            if (j > 0 && kind != serve::kGreeks) return set_error(BS_ERR_INVALID, "not a Greeks ticket");
+            e = hipMemcpy(dst[j], c->exec.arena + off + oo[j], (size_t)n * 4, hipMemcpyDeviceToHost);
+        }
+        return e == hipSuccess ? BS_OK : hip_error(e, "bs_result_copy");
+    });
 }
 
 int bs_result_release(bs_ctx_t ctx, bs_ticket_t ticket) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    return status(c->loop->release(ticket), "bs_result_release");
+    return guarded("bs_result_release", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        return status(c->loop->release(ticket), "bs_result_release");
+    });
 }
 
 int bs_cancel(bs_ctx_t ctx, bs_ticket_t ticket) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    return status(c->loop->cancel(ticket), "bs_cancel: ticket not queued");
+    return guarded("bs_cancel", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        return status(c->loop->cancel(ticket), "bs_cancel: ticket not queued");
+    });
 }
 
 int bs_ticket_times(bs_ctx_t ctx, bs_ticket_t ticket, int64_t* submit_ns, int64_t* done_ns) {
-    bs_ctx_s* c = ctx;
-    if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
-    return status(c->loop->times(ticket, submit_ns, done_ns), "unknown ticket");
+    return guarded("bs_ticket_times", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c) return set_error(BS_ERR_INVALID, "ctx is NULL");
+        return status(c->loop->times(ticket, submit_ns, done_ns), "unknown ticket");
+    });
 }
 
 int bs_ctx_stats(bs_ctx_t ctx, bs_serve_stats* out) {
-    bs_ctx_s* c = ctx;
-    if (!c || !out) return set_error(BS_ERR_INVALID, "ctx or stats is NULL");
-    serve::Stats st;
-    serve::LoopTimes lt;
-    size_t cap, used, peak;
-    c->loop->stats(&st, &cap, &used, &peak, &lt);
-    out->arena_bytes = (int64_t)cap;
-    out->arena_used = (int64_t)used;
-    out->arena_peak = (int64_t)peak;
-    out->books_resident = st.books_resident;
-    out->requests_queued = st.requests_queued;
-    out->requests_running = st.requests_running;
-    out->requests_done = st.requests_done;
-    out->requests_failed = st.requests_failed;
-    out->requests_cancelled = st.requests_cancelled;
-    out->launches = st.launches;
-    out->options_priced = st.options_priced;
-    out->admission_waits = st.admission_waits;
-    out->queue_ns_total = st.queue_ns;
-    out->batch_ns_total = lt.batch;
-    out->dispatch_admit_ns = lt.admit;
-    out->dispatch_launch_ns = lt.launch;
-    out->dispatch_retire_ns = lt.retire;
-    out->gpu_idle_ns = lt.idle;
-    out->idle_launches = lt.idle_launches;
-    return BS_OK;
+    return guarded("bs_ctx_stats", [&]() -> int {
+        bs_ctx_s* c = ctx;
+        if (!c || !out) return set_error(BS_ERR_INVALID, "ctx or stats is NULL");
+        serve::Stats st;
+        serve::LoopTimes lt;
+        size_t cap, used, peak;
+        c->loop->stats(&st, &cap, &used, &peak, &lt);
+        out->arena_bytes = (int64_t)cap;
+        out->arena_used = (int64_t)used;
+        out->arena_peak = (int64_t)peak;
+        out->books_resident = st.books_resident;
+        out->requests_queued = st.requests_queued;
+        out->requests_running = st.requests_running;
+        out->requests_done = st.requests_done;
+        out->requests_failed = st.requests_failed;
+        out->requests_cancelled = st.requests_cancelled;
+        out->launches = st.launches;
+        out->options_priced = st.options_priced;
+        out->admission_waits = st.admission_waits;
+        out->queue_ns_total = st.queue_ns;
+        out->batch_ns_total = lt.batch;
+        out->dispatch_admit_ns = lt.admit;
+        out->dispatch_launch_ns = lt.launch;
+        out->dispatch_retire_ns = lt.retire;
+        out->gpu_idle_ns = lt.idle;
+        out->idle_launches = lt.idle_launches;
+        return BS_OK;
+    });
 }
 
 }  // extern "C"
