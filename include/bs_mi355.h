@@ -108,11 +108,12 @@ int bs_price_i32type_f32(const float *s, const float *k, const float *r, const f
  * BS:607-658, bs_thread's): prices the book `passes` times on `stream`, each
  * pass starting after the previous one ends, with the same output as calling
  * bs_price_f32 (otype_bytes = 1, uint8 types) or bs_price_i32type_f32
- * (otype_bytes = 4, int32 types) `passes` times.  Books of up to 2^20 options
- * are launch-bound: their passes are one hipGraph of `passes` kernel nodes,
- * captured on the first call for these arrays, n, passes and device, and
- * replayed after that (the library keeps the 16 most recently used).  Larger
- * books launch pass by pass. */
+ * (otype_bytes = 4, int32 types) `passes` times.  For books of up to 2^25
+ * options the passes are one hipGraph of `passes` kernel nodes, captured on
+ * the first call for these arrays, n, passes and device, and replayed after
+ * that (the library keeps the 16 most recently used): small books are bound
+ * by launch cost, and larger ones lose less to the cache maintenance between
+ * separately launched passes.  Larger books launch pass by pass. */
 int bs_price_passes_f32(const float *s, const float *k, const float *r, const float *v,
                         const float *t, const void *otype, int otype_bytes, float *price,
                         int64_t n, int passes, bs_stream_t stream);

@@ -2,7 +2,7 @@
 MI355X, through the C ABI.
 
 * Output bit-identical to one bs_price_f32 / bs_price_i32type_f32 pass, for
-  books on both sides of the graph threshold (2^20 options), 1 to 100 passes,
+  books on both sides of the graph threshold (2^25 options), 1 to 100 passes,
   both type widths, misaligned arrays.
 * Graph replay: the same call replayed with the inputs rewritten between
   replays and eager single passes interleaved; every replay equals the eager
@@ -32,8 +32,8 @@ def eager(capi, db, n):
     return got
 
 
-@pytest.mark.parametrize("n,passes", [(1, 100), (4096, 100), (100_003, 7), (1 << 20, 3),
-                                      ((1 << 20) + 1, 2), (4097, 1)])
+@pytest.mark.parametrize("n,passes", [(1, 100), (4096, 100), (100_003, 7), (10_000_000, 3),
+                                      (1 << 25, 2), ((1 << 25) + 1, 2), (4097, 1)])
 @pytest.mark.parametrize("type_dtype", [np.uint8, np.int32])
 def test_same_bits_as_single_passes(gpu, n, passes, type_dtype):
     book = books.make_book(n, books.SEED_BASE + 100 + n % 89)
