@@ -198,9 +198,15 @@ int bs_event_elapsed_ms(bs_event_t start, bs_event_t stop, float *ms);
  * (BS:664-960).  A context owns one HBM arena of `hbm_budget_bytes`; books
  * registered into it stay resident; requests against resident books are
  * queued, admitted when their output buffers fit in the arena, fused with the
- * other queued requests into one batch (a fused kernel per request kind,
- * launched directly or replayed from a captured hipGraph), and completed in
- * the order their batches finish.  A context is thread-safe. */
+ * other queued requests of their lane into one batch (a fused kernel per
+ * request kind, launched directly or replayed from a captured hipGraph), and
+ * completed in the order their batches finish.  Requests of at most 64K
+ * options form a lane of their own (bs_serve_opts.small_request_options).
+ * Every call on a context may come from any thread at once, except
+ * bs_ctx_destroy, which must not overlap any other call on that context: it
+ * finishes the batches in flight, fails the queued requests with
+ * BS_ERR_SHUTDOWN and frees the arena (result pointers from
+ * bs_result_device die with it). */
 typedef struct bs_ctx_s *bs_ctx_t;
 typedef int64_t bs_book_t;
 typedef int64_t bs_ticket_t;
