@@ -10,27 +10,25 @@
 // reference's repack at blackscholes.c:747-767), then NUM_RUNS = 100 passes of
 // the price map run on the GPU (the reference's ROI, blackscholes.c:782-913).
 // <nthreads> is accepted for command-line compatibility; the GPU map does not
-// use host threads.  --check counts prices off the DGrefval column by 1e-4 or
+// use host threads (the text formats are read and written on up to 16).  --check counts prices off the DGrefval column by 1e-4 or
 // more, as the reference's ERR_CHK build does (blackscholes.c:333-340).
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "bs_mi355.h"
+#include "parsec_io.hpp"
 
 namespace {
 
-// The reference's OptionData (blackscholes.c:89-100): 36 bytes.
-struct OptionData {
-    float s, strike, r, divq, v, t;
-    char OptionType;
-    float divs, DGrefval;
-};
-static_assert(sizeof(OptionData) == 36, "OptionData layout");
+using parsec_io::OptionData;
 
 int fail(const char* what, int rc) {
     fprintf(stderr, "ERROR: %s: %s (%s)\n", what, bs_status_string(rc), bs_last_error());
@@ -59,33 +57,20 @@ int main(int argc, char** argv) {
     const char* inputFile = pos[1];
     const char* outputFile = pos[2];
 
-    FILE* file = fopen(inputFile, "r");
-    if (!file) {
-        printf("ERROR: Unable to open file `%s'.\n", inputFile);
+    // the text formats are read and written on the host's threads (the
+    // conversions are the reference's; see parsec_io.hpp)
+    const int io_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<OptionData> data;
+    std::string io_err;
+    if (!parsec_io::read_options(inputFile, &data, io_threads, &io_err)) {
+        printf("ERROR: %s\n", io_err.c_str());
         return 1;
     }
-    int numOptions = 0;
-    if (fscanf(file, "%i", &numOptions) != 1 || numOptions < 0) {
-        printf("ERROR: Unable to read from file `%s'.\n", inputFile);
-        fclose(file);
-        return 1;
-    }
+    const int numOptions = (int)data.size();
     if (nThreads > numOptions) {
         printf("WARNING: Not enough work, reducing number of threads to match number of options.\n");
         nThreads = numOptions;
     }
-    std::vector<OptionData> data((size_t)numOptions);
-    for (int i = 0; i < numOptions; ++i) {
-        OptionData& d = data[i];
-        memset(&d, 0, sizeof(d));
-        if (fscanf(file, "%f %f %f %f %f %f %c %f %f", &d.s, &d.strike, &d.r, &d.divq, &d.v, &d.t,
-                   &d.OptionType, &d.divs, &d.DGrefval) != 9) {
-            printf("ERROR: Unable to read from file `%s'.\n", inputFile);
-            fclose(file);
-            return 1;
-        }
-    }
-    fclose(file);
     printf("Num of Options: %d\n", numOptions);
     printf("Num of Runs: %d\n", passes);
     printf("Size of data: %zu\n", (size_t)numOptions * (sizeof(OptionData) + sizeof(int)));
@@ -126,25 +111,8 @@ int main(int argc, char** argv) {
         bs_stream_destroy(st);
     }
 
-    file = fopen(outputFile, "w");
-    if (!file) {
-        printf("ERROR: Unable to open file `%s'.\n", outputFile);
-        return 1;
-    }
-    if (fprintf(file, "%i\n", numOptions) < 0) {
-        printf("ERROR: Unable to write to file `%s'.\n", outputFile);
-        fclose(file);
-        return 1;
-    }
-    for (int i = 0; i < numOptions; ++i) {
-        if (fprintf(file, "%.18f\n", prices[i]) < 0) {
-            printf("ERROR: Unable to write to file `%s'.\n", outputFile);
-            fclose(file);
-            return 1;
-        }
-    }
-    if (fclose(file) != 0) {
-        printf("ERROR: Unable to close file `%s'.\n", outputFile);
+    if (!parsec_io::write_prices(outputFile, prices.data(), numOptions, io_threads, &io_err)) {
+        printf("ERROR: %s\n", io_err.c_str());
         return 1;
     }
     if (check) {
