@@ -287,6 +287,60 @@ def serve_bench(result: dict) -> dict:
     return out
 
 
+SW_TRIALS = 1_000_000    # PARSEC's native run: 128 swaptions, 1,000,000 trials each
+SW_COUNT = 128
+
+
+def swaptions_bench(no_cpu: bool, calls: int = 3) -> dict:
+    """The swaptions map on PARSEC's native input: every path of every
+    swaption on the GPU (bs_swaptions_host_f64), timed per whole call (host
+    records in, prices out) and by HIP events on its stream; parity of the
+    whole portfolio at 100,000 trials against the oracle, whose static-block
+    run over 16 host threads is the CPU baseline."""
+    from p3arsec_amd import swaptions as W
+    sws, base = W.parsec_portfolio(SW_COUNT)
+    paths = SW_COUNT * ((SW_TRIALS + W.BLOCK_SIZE - 1) // W.BLOCK_SIZE * W.BLOCK_SIZE)
+    W.price(sws[:4], 10_000, base)            # first call: code object load
+    st, e0, e1 = capi.Stream(), capi.Event(), capi.Event()
+    wall, gpu = [], []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        e0.record(st.handle)
+        p, se = W.price(sws, SW_TRIALS, base, stream=st.handle)
+        e1.record(st.handle)
+        e1.sync()
+        wall.append(time.perf_counter() - t0)
+        gpu.append(e0.elapsed_ms(e1))
+    out = {"workload": f"{SW_COUNT} swaptions x {SW_TRIALS} trials (PARSEC native input): "
+                       "HJM, 11 steps, 3 factors, FP64",
+           "seconds_per_call": min(wall), "gpu_ms_per_call": min(gpu),
+           "paths_per_s": paths / min(wall), "swaptions_per_s": SW_COUNT / min(wall),
+           "price_min": float(p.min()), "price_max": float(p.max())}
+    st.destroy()
+    e0.destroy()
+    e1.destroy()
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyswaptions
+        trials = 100_000
+        gp, ge = W.price(sws, trials, base)
+        cp, ce, sec, used = pyswaptions.price_batch(sws, base, trials, W.BLOCK_SIZE, 0)
+        rel = np.abs(gp - cp) / np.maximum(np.abs(cp), 1e-300)
+        out["parity_100k"] = {"max_rel_price_diff": float(rel[cp > 0].max()),
+                              "max_abs_price_diff": float(np.abs(gp - cp).max()),
+                              "max_rel_stderr_diff": float((np.abs(ge - ce) /
+                                                            np.maximum(ce, 1e-300))[ce > 0].max())}
+        cpaths = SW_COUNT * trials
+        out["cpu_baseline"] = {"value": cpaths / sec, "unit": "paths/s", "cores": used,
+                               "kind": "port",
+                               "sample": f"{SW_COUNT} swaptions x {trials} trials, static blocks "
+                                         f"over swaptions, {used} pinned threads, "
+                                         f"oracle/swaptions_oracle.c (HJM_Swaption_Blocking "
+                                         f"restated, every time step simulated), {sec:.2f} s",
+                               "gpu_speedup": (paths / min(wall)) / (cpaths / sec)}
+    return out
+
+
 def only_one(args, L, host, dbooks, outs, streams) -> int:
     """Runs one kernel for --warmup + --steps passes (profiler runs)."""
     n, nb = host.n, len(dbooks)
@@ -344,8 +398,9 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--no-swaptions", action="store_true")
     ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy", "portfolio",
-                                       "portfolio_greeks"],
+                                       "portfolio_greeks", "swaptions"],
                     help="time one kernel only (for profiler runs)")
     ap.add_argument("--one-stream", action="store_true",
                     help="with --only: every pass on one stream")
@@ -363,6 +418,10 @@ def main() -> int:
         print("no HIP device", file=sys.stderr)
         return 1
     capi.call("bs_set_device", 0)
+    if args.only == "swaptions":
+        r = swaptions_bench(True, calls=max(1, args.steps))
+        print(json.dumps({"kernel": "swaptions", **r}), flush=True)
+        return 0
     n, nb = args.n, args.books
     log(f"bench: generating {n} options")
     host = books.make_book(n, books.SEED_BASE + 2)
@@ -554,6 +613,13 @@ def main() -> int:
             result["serve"] = serve_bench(result)
         except Exception as e:  # recorded, so the headline line still prints
             section_failed(result, "serve", e)
+
+    if not args.no_extra and not args.no_swaptions:
+        try:
+            log("bench: swaptions")
+            result["swaptions"] = swaptions_bench(args.no_cpu)
+        except Exception as e:  # recorded, so the headline line still prints
+            section_failed(result, "swaptions", e)
 
     if not args.no_cpu:
         try:

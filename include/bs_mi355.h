@@ -171,6 +171,41 @@ int bs_bind_globals(int num_options, int nthreads, float *sptprice, float *strik
                     float *prices);
 int bs_thread_compat(void *tid_ptr);
 
+/* ---- swaptions: HJM Monte-Carlo -------------------------------------------
+ * The suite's other computational-finance map (SW = parsec-ff/pkgs/apps/
+ * swaptions/src).  One record per swaption, the fields of the reference's
+ * `parm` (HJM_Securities.cpp:275-296) that HJM_Swaption_Blocking reads
+ * (HJM_Swaption_Blocking.cpp:20-40); yield and factors are HOST pointers. */
+typedef struct {
+    double strike;           /* dStrike                                        */
+    double compounding;      /* dCompounding: 0 = continuous, else periods/yr  */
+    double maturity;         /* dMaturity: years to the swap's start           */
+    double tenor;            /* dTenor: years of the swap                      */
+    double payment_interval; /* dPaymentInterval: years between payments       */
+    double years;            /* dYears: HJM horizon                            */
+    int32_t n_steps;         /* iN: HJM time points, 2..64                     */
+    int32_t n_factors;       /* iFactors: 1..8, the same for every record      */
+    const double *yield;     /* pdYield[n_steps]                               */
+    const double *factors;   /* ppdFactors: n_factors rows of n_steps - 1      */
+} bs_swaption_t;
+
+/* Replaces the FastFlow ParallelFor over swaptions (SW HJM_Securities.cpp:
+ * 311-323) and each call HJM_Swaption_Blocking(pdSwaptionPrice, ...,
+ * iRndSeed = seed + i, lTrials = trials, BLOCKSIZE = block_size)
+ * (HJM_Swaption_Blocking.cpp:20-222): price[i] and std_error[i] are swaption
+ * i's Monte-Carlo mean and standard error, FP64, from the same random
+ * stream as the reference (ceil(trials / block_size) * block_size paths,
+ * sums divided by trials).  Every path of every swaption runs on the GPU.
+ * Host arrays; synchronous on `stream`.  Limits: trials in [1, 2^40],
+ * block_size >= 1, seed in [0, 2^42) and seed + n + paths * (n_steps - 1) *
+ * n_factors below 2^42, the swap's start step x n_factors <= 64; a record
+ * the reference would index out of its arrays with (maturity past the
+ * horizon, tenor past the curve, payment interval under one step) is
+ * BS_ERR_INVALID. */
+int bs_swaptions_host_f64(const bs_swaption_t *sw, int64_t n, int64_t trials, int64_t seed,
+                          int32_t block_size, double *price, double *std_error,
+                          bs_stream_t stream);
+
 /* ---- device plumbing for hosts without a GPU runtime of their own ------- */
 int bs_device_count(int *count);
 int bs_set_device(int device);

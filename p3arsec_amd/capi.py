@@ -66,6 +66,16 @@ class ServeStats(ctypes.Structure):
         "gpu_idle_ns", "idle_launches")]
 
 
+class Swaption(ctypes.Structure):
+    """bs_swaption_t: one swaption's parameters (host pointers)."""
+    _fields_ = [("strike", ctypes.c_double), ("compounding", ctypes.c_double),
+                ("maturity", ctypes.c_double), ("tenor", ctypes.c_double),
+                ("payment_interval", ctypes.c_double), ("years", ctypes.c_double),
+                ("n_steps", ctypes.c_int32), ("n_factors", ctypes.c_int32),
+                ("yield_", ctypes.POINTER(ctypes.c_double)),
+                ("factors", ctypes.POINTER(ctypes.c_double))]
+
+
 # name -> (argtypes, restype); restype int unless given
 _SIGS = {
     "bs_status_string": ([_i32], ctypes.c_char_p),
@@ -123,6 +133,8 @@ _SIGS = {
     "bs_cancel": ([_vp, _i64], _i32),
     "bs_ticket_times": ([_vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _i32),
     "bs_ctx_stats": ([_vp, ctypes.POINTER(ServeStats)], _i32),
+    "bs_swaptions_host_f64": ([ctypes.POINTER(Swaption), _i64, _i64, _i64, ctypes.c_int32,
+                               _vp, _vp, _vp], _i32),
 }
 
 _lib = None

@@ -81,3 +81,47 @@ def test_scalar_and_thread_compat_arguments():
     tid = ctypes.c_int(5)
     assert L.bs_bind_globals(10, 2, *([None] * 7)) == 0
     assert L.bs_thread_compat(ctypes.byref(tid)) == -1       # tid >= nthreads
+
+
+def test_swaptions_argument_errors_before_device_work():
+    from p3arsec_amd import swaptions as W
+    L = capi.load()
+    sws, _ = W.parsec_portfolio(2)
+
+    def rc(sws_, trials=1000, seed=5, block=16, n=None):
+        recs, keep = W._records(sws_)
+        p = np.empty(max(len(sws_), 1))
+        e = np.empty(max(len(sws_), 1))
+        r = L.bs_swaptions_host_f64(recs, len(sws_) if n is None else n, trials, seed, block,
+                                    p.ctypes.data, e.ctypes.data, None)
+        del keep
+        return r
+
+    def mod(**kw):
+        s = dict(sws[0])
+        s.update(kw)
+        return [s]
+
+    assert L.bs_swaptions_host_f64(None, 0, 10, 0, 16, None, None, None) == 0   # nothing to do
+    assert L.bs_swaptions_host_f64(None, 2, 10, 0, 16, None, None, None) == -1
+    assert rc(sws, n=-1) == -1
+    assert rc(sws, trials=0) == -1
+    assert rc(sws, block=0) == -1
+    assert rc(sws, seed=-1) == -1
+    assert rc(sws, seed=2 ** 42) == -1
+    assert rc(sws, trials=2 ** 38) == -1               # counters past 2^42
+    assert b"2^42" in L.bs_last_error()
+    assert rc(mod(maturity=30.0)) == -1                 # start past the horizon
+    assert b"horizon" in L.bs_last_error()
+    assert rc(mod(interval=0.01)) == -1                 # payments under one step
+    assert rc(mod(tenor=40.0)) == -1                    # swap past the curve
+    assert rc(mod(years=float("nan"))) == -1
+    assert rc(mod(years=0.0)) == -1
+    one = mod(**{"yield": np.array([0.1])}, factors=np.zeros((3, 0)))
+    assert rc(one) == -1                                # n_steps < 2
+    assert rc(mod(factors=np.zeros((9, 10)))) == -1     # 9 factors
+    assert rc([sws[0], dict(sws[1], factors=W.PARSEC_FACTORS[:2])]) == -1   # mixed n_factors
+    assert b"same n_factors" in L.bs_last_error()
+    long_ = mod(**{"yield": 0.05 + 0.001 * np.arange(40)}, factors=np.full((8, 39), 0.01),
+                years=40.0, maturity=9.0, tenor=2.0)
+    assert rc(long_) == -1                              # start 9 x 8 factors > 64 draws
