@@ -11,6 +11,8 @@
 //                                                       blackscholes.c:315-348
 //   ff::ParallelFor::parallel_for / parallel_for_thid   fastflow/ff/parallel_for.hpp:111-317
 //   skepu2::Vector<T>(T*, size_t, bool), skepu2 Map<7>  blackscholes_skepu_omp.cpp:710-728
+//   int HJM_Swaption_Blocking(FTYPE* pdSwaptionPrice, ...)
+//                              swaptions/src/HJM_Swaption_Blocking.cpp:20-222
 // Every call below runs on the GPU through the C ABI (include/bs_mi355.h);
 // nothing here prices on the host.  Errors throw bs_compat::Error (the C ABI
 // itself never throws).
@@ -20,6 +22,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "bs_mi355.h"
 
@@ -62,6 +65,28 @@ inline void BlkSchlsEqEuroNoDiv(float* OptionPrice, int numOptions, float* sptpr
     check(bs_price_host_f32(sptprice, strike, rate, volatility, time, otype, OptionPrice,
                             numOptions, 1, 0),
           "BlkSchlsEqEuroNoDiv(batch)");
+}
+
+// HJM_Swaption_Blocking with the reference's signature (FTYPE = double,
+// HJM_Swaption_Blocking.cpp:20-40): one swaption's Monte-Carlo price and
+// standard error into pdSwaptionPrice[0..1], every path on the GPU.  Returns
+// 1 as the reference does; a record the reference cannot price throws.  tid
+// (the reference's instrumentation thread id) is unused.
+inline int HJM_Swaption_Blocking(double* pdSwaptionPrice, double dStrike, double dCompounding,
+                                 double dMaturity, double dTenor, double dPaymentInterval,
+                                 int iN, int iFactors, double dYears, const double* pdYield,
+                                 double** ppdFactors, long iRndSeed, long lTrials,
+                                 int BLOCKSIZE, int /*tid*/ = 0) {
+    // ppdFactors is the reference's row-pointer matrix: gather it row-major
+    std::vector<double> fac((size_t)iFactors * (iN > 1 ? iN - 1 : 0));
+    for (int k = 0; k < iFactors && iN > 1; ++k)
+        for (int j = 0; j < iN - 1; ++j) fac[(size_t)k * (iN - 1) + j] = ppdFactors[k][j];
+    const bs_swaption_t sw{dStrike, dCompounding, dMaturity, dTenor, dPaymentInterval, dYears,
+                           iN, iFactors, pdYield, fac.data()};
+    check(bs_swaptions_host_f64(&sw, 1, lTrials, iRndSeed, BLOCKSIZE, &pdSwaptionPrice[0],
+                                &pdSwaptionPrice[1], nullptr),
+          "HJM_Swaption_Blocking");
+    return 1;
 }
 
 // bs_thread drop-in over the caller's globals (blackscholes.c:102-113).

@@ -30,7 +30,10 @@
  *                        sigma_k[l] Z_k(j) and the running products of
  *                        exp(-r dt), over BLOCKSIZE paths at a time.
  * Those restatements are not pinned against PARSEC's own sources (absent);
- * the tests pin them against what the mathematics fixes instead: Moro
+ * the tests pin them against what the mathematics fixes instead.  Parity
+ * unpinned: the generator's counter hash (1513517), its scale
+ * (4.656612875e-10) and the draw order (path, time step, factor), which no
+ * check can tie to PARSEC's sources or outputs here.  Pinned: Moro
  * against the exact inverse normal, Schrage's step against direct modular
  * arithmetic and Park-Miller's published check value, the forward curve and
  * drift identities, and deep in-the-money prices against their closed form

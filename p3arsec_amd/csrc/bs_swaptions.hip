@@ -15,9 +15,12 @@
 // on the curve f(m, .) at the swaption's start m.  The payoff reads only the
 // short rates before m and the curve at m, so the kernel follows each forward
 // chain d (f(0,d) -> f(1,d-1) -> ...) through steps 1..min(d, m) only, with
-// the reference's additions in the reference's order: the prices are the
-// reference's up to the device exp/log (last-bit differences) and the order
-// of the final sums.  With the reference's parameters (11 steps, maturity 1
+// the reference's additions in the reference's order, and takes the swap
+// curve's exponentials only up to the swap's last cash flow (past it the
+// reference multiplies its discount factors by zero cash flows; a guard sends
+// any path where those factors could be non-finite through the full
+// computation).  The prices are the reference's up to the device exp/log
+// (last-bit differences) and the order of the final sums.  With the reference's parameters (11 steps, maturity 1
 // year on a 5-20 year horizon) m is 1 or 2 of the 10 steps.
 //
 // Generator: the reference's RanUnif (PARSEC 3.0 RanUnif.cpp, not in the
@@ -60,8 +63,10 @@ struct SwHead {
     int64_t seed;        // counter of path 0's first draw
     double dt, sdt, dts; // HJM step, its square root, the swap curve's step
     int32_t n, m, L, off;
+    int32_t last;        // index of the swap's last cash flow (pay[l] = 0 past it)
+    int32_t pad;
 };
-static_assert(sizeof(SwHead) == 48, "SwHead layout");
+static_assert(sizeof(SwHead) == 56, "SwHead layout");
 
 __device__ __forceinline__ uint64_t mod_m31(uint64_t x) {   // x < 2^63
     x = (x & kM31) + (x >> 31);
@@ -97,10 +102,20 @@ __device__ __forceinline__ double moro_inv(double u) {
 }
 
 // Discounted payoff of path p.  F factors; the swaption starts at step m <= MAXM.
-template <int F, int MAXM>
+//
+// kAll = true is the reference's arithmetic in full.  kAll = false skips the
+// swap-curve exponentials past the last cash flow: those discount factors
+// only ever multiply pay[l] = 0, so the terms they make are +0 and leave the
+// fixed leg unchanged -- provided every one of them is finite.  That holds
+// when the rates past the last cash flow are finite, the discount factor at
+// the last cash flow is at most 1e300 and the positive exponents past it sum
+// to at most 1 (the factors then stay below 1e300 e); *exact reports it, and
+// the caller recomputes the path with kAll = true when it does not hold.
+template <int F, int MAXM, bool kAll>
 __device__ __forceinline__ double path_payoff(const SwHead& h, const double* __restrict__ q,
-                                              int64_t p) {
+                                              int64_t p, bool* exact) {
     const int n = h.n, m = h.m, L = h.L, w = n - 1;
+    const int last = kAll ? L : h.last;
     const double* f0 = q;
     const double* mudt = q + n;
     const double* sigma = q + n + w;
@@ -113,7 +128,8 @@ __device__ __forceinline__ double path_payoff(const SwHead& h, const double* __r
 #pragma unroll
         for (int k = 0; k < F; ++k) z[s][k] = moro_inv(ran_unif(c0 + s * F + k));
     }
-    double df = 1.0, sdf = 1.0, leg = 0.0;
+    double df = 1.0, sdf = 1.0, leg = 0.0, grow = 0.0;
+    bool finite = true;
     const int dend = min(n, m + L);
     for (int d = 0; d < dend; ++d) {
         double x = f0[d];
@@ -131,12 +147,22 @@ __device__ __forceinline__ double path_payoff(const SwHead& h, const double* __r
             df *= exp(-x * h.dt);
         } else {
             const int l = d - m;
-            leg += pay[l] * sdf;
-            if (l < L - 1) sdf *= exp(-x * h.dts);
+            if (l <= last) leg += pay[l] * sdf;
+            if (l < L - 1) {
+                const double y = -x * h.dts;
+                if (l < last) {
+                    sdf *= exp(y);
+                } else {
+                    finite = finite && fabs(y) <= 1e300;   // false for inf and NaN
+                    grow += y > 0 ? y : 0;
+                }
+            }
         }
     }
     // curve points past the simulated horizon are zero rates: factor exp(-0) = 1
-    for (int l = n - m; l < L; ++l) leg += pay[l] * sdf;
+    for (int l = n - m; l < L; ++l)
+        if (l <= last) leg += pay[l] * sdf;
+    *exact = kAll || (finite && grow <= 1.0 && sdf <= 1e300);
     const double payoff = leg - 1.0 > 0 ? leg - 1.0 : 0;
     return payoff * df;
 }
@@ -156,7 +182,9 @@ __global__ __launch_bounds__(kSwThreads) void k_sw_paths(const SwHead* __restric
     for (int j = 0; j < kSwPer; ++j) {
         const int64_t p = (chunk * kSwPer + j) * kSwThreads + threadIdx.x;
         if (p < paths) {
-            const double v = path_payoff<F, MAXM>(h, q, p);
+            bool exact;
+            double v = path_payoff<F, MAXM, false>(h, q, p, &exact);
+            if (!exact) v = path_payoff<F, MAXM, true>(h, q, p, &exact);
             sum += v;
             sumsq += v * v;
         }
@@ -286,6 +314,8 @@ int pack_swaption(const bs_swaption_t& s, int64_t i, int64_t seed, SwHead* h,
     h->m = m;
     h->L = L;
     h->off = (int32_t)pool->size();
+    h->last = points;
+    h->pad = 0;
     const size_t base = pool->size();
     pool->resize(base + n + w + (size_t)F * w + L, 0.0);
     double* f0 = pool->data() + base;

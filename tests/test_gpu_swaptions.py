@@ -48,6 +48,9 @@ def sw(**kw):
 
 def check(got, want, tag):
     (gp, ge), (op, oe) = got, want
+    if math.isnan(op) or math.isinf(op):
+        assert gp == op or (math.isnan(gp) and math.isnan(op)), (tag, gp, op)
+        return 0.0
     assert abs(gp - op) <= 1e-12 * abs(op) + 1e-13, (tag, gp, op)
     # When the payoffs barely vary (a swaption starting at t = 0 pays the same
     # on every path), sumsq - sum^2/n is rounding noise in both, its sign and
@@ -86,6 +89,9 @@ SHAPES = [   # strikes near the money: 20-40% of paths pay
     dict(n=64, F=1, years=16.0, maturity=10.0, tenor=5.0, strike=0.41),    # 64 steps, 40 draws
     dict(n=2, F=3, years=2.0, maturity=0.0, tenor=1.0, strike=0.3,
          factors=W.PARSEC_FACTORS[:, :1]),                                 # smallest curve
+    # strongly negative rates: the discount factors past the last cash flow
+    # grow, so every path takes the kernel's full computation
+    dict(years=11.0, tenor=2.0, strike=-0.3, **{"yield": np.full(11, -0.4)}),
 ]
 
 
@@ -124,6 +130,19 @@ def test_one_trial_and_single_path_blocks(gpu):
     for trials, block in ((1, 1), (1, 16), (2, 1)):
         got = W.price([s], trials, 99, block)
         check((got[0][0], got[1][0]), S.price_blocking(s, 99, trials, block), (trials, block))
+
+
+def test_zero_draw_gives_the_reference_nan_and_inf(gpu):
+    """Counter 2^31 - 1 draws u = 0, whose inverse normal is -inf: that path's
+    rates become infinite or NaN (0 x inf where a factor is 0).  The map
+    follows the reference through it: same price, NaN or inf included."""
+    sws, _ = W.parsec_portfolio(4)
+    m31 = 2147483647
+    for i, s in enumerate(sws):
+        for back in (1, 5, 29, 31, 95):
+            seed = m31 - back
+            got = W.price([s], 64, seed)
+            check((got[0][0], got[1][0]), S.price_blocking(s, seed, 64), (i, back))
 
 
 def test_native_workload(gpu):
