@@ -29,9 +29,10 @@
 // identity 2^31 = 1 (mod 2^31-1) in 64-bit integers, which gives the same
 // residues as its Schrage factorisation.
 //
-// Kernels (the caller's stream): k_sw_paths, one path per work-item, kSwPer
-// paths per work-item, each block's sums of payoff and payoff^2 folded in a
-// fixed order into one partial; k_sw_final, one block per swaption, adds its
+// Kernels (the caller's stream): k_sw_paths, one path at a time per work-item,
+// kSwPer paths each, Moro's tail evaluated for a whole wave's draws at once
+// (draws_compact), each block's sums of payoff and payoff^2 folded in a fixed
+// order into one partial; k_sw_final, one block per swaption, adds its
 // partials in a fixed order and forms the mean and standard error as the
 // reference does.  Results are bit-reproducible for given inputs.  The
 // arithmetic is FP64 (the reference's FTYPE); this file is compiled without
@@ -53,6 +54,7 @@ constexpr int kSwPer = 4;                 // paths per work-item
 constexpr int kSwMaxSteps = 64;           // n_steps
 constexpr int kSwMaxFactors = 8;          // n_factors
 constexpr int kSwMaxDraws = 64;           // start index x n_factors
+constexpr int kSwCompactDraws = 12;       // register forms up to this many draws compact Moro's tail
 constexpr int64_t kSwCounterLimit = int64_t(1) << 42;
 constexpr uint64_t kM31 = 2147483647u;
 
@@ -82,17 +84,20 @@ __device__ __forceinline__ double ran_unif(int64_t c) {
 }
 
 // Moro's inverse normal (Moro 1995; Glasserman, Monte Carlo Methods in
-// Financial Engineering, fig. 2.13), as PARSEC's CumNormalInv evaluates it.
-__device__ __forceinline__ double moro_inv(double u) {
+// Financial Engineering, fig. 2.13), as PARSEC's CumNormalInv evaluates it:
+// a rational function of x = u - 0.5 for |x| < 0.42, a Chebyshev series in
+// log(-log(min(u, 1 - u))) outside.
+__device__ __forceinline__ double moro_central(double x) {
+    const double r = x * x;
+    const double num = x * (((-25.44106049637 * r + 41.39119773534) * r + -18.61500062529) * r +
+                            2.50662823884);
+    const double den = (((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r +
+                        -8.47351093090) * r + 1.0;
+    return num / den;
+}
+
+__device__ __forceinline__ double moro_tail(double u) {
     const double x = u - 0.5;
-    if (fabs(x) < 0.42) {
-        const double r = x * x;
-        const double num = x * (((-25.44106049637 * r + 41.39119773534) * r + -18.61500062529) * r +
-                                2.50662823884);
-        const double den = (((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r +
-                            -8.47351093090) * r + 1.0;
-        return num / den;
-    }
     double t = x > 0.0 ? 1.0 - u : u;
     t = log(-log(t));
     t = 0.3374754822726147 + t * (0.9761690190917186 + t * (0.1607979714918209 +
@@ -101,7 +106,71 @@ __device__ __forceinline__ double moro_inv(double u) {
     return x < 0.0 ? -t : t;
 }
 
-// Discounted payoff of path p.  F factors; the swaption starts at step m <= MAXM.
+__device__ __forceinline__ double moro_inv(double u) {
+    const double x = u - 0.5;
+    return fabs(x) < 0.42 ? moro_central(x) : moro_tail(u);
+}
+
+// The normal draws of path p: z[s][k] = Z_k(s + 1) for s < m.
+template <int F, int MAXM>
+__device__ __forceinline__ void draws_direct(const SwHead& h, int64_t p, double (&z)[MAXM][F]) {
+    const int64_t c0 = h.seed + p * (int64_t)((h.n - 1) * F);
+#pragma unroll
+    for (int s = 0; s < MAXM; ++s) {
+        if (s >= h.m) break;
+#pragma unroll
+        for (int k = 0; k < F; ++k) z[s][k] = moro_inv(ran_unif(c0 + s * F + k));
+    }
+}
+
+// The same draws, with Moro's tail (16% of draws, but taken by nearly every
+// wave at every draw when each lane branches on its own) compacted: the lanes
+// write the uniforms that need it to their wave's LDS list (ballot + mbcnt),
+// every lane then evaluates the tail for list entries lane, lane + 64, ...,
+// and each owner reads its results back.  The same operations on the same
+// values as draws_direct.  Every thread of the block must call it (it
+// synchronises the block); wl is this wave's MAXM F 64 doubles.
+template <int F, int MAXM>
+__device__ __forceinline__ void draws_compact(const SwHead& h, int64_t p, double (&z)[MAXM][F],
+                                              double* wl) {
+    const int64_t c0 = h.seed + p * (int64_t)((h.n - 1) * F);
+    const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    int pos[MAXM][F];
+    bool tail[MAXM][F];
+    int count = 0;   // tail entries of this wave so far (uniform)
+#pragma unroll
+    for (int s = 0; s < MAXM; ++s) {
+        const bool live = s < h.m;   // uniform
+#pragma unroll
+        for (int k = 0; k < F; ++k) {
+            const double u = live ? ran_unif(c0 + s * F + k) : 0.25;
+            const double x = u - 0.5;
+            const bool t = !(fabs(x) < 0.42);
+            z[s][k] = moro_central(x);
+            const uint64_t b = __ballot(t);
+            pos[s][k] = count + (int)__builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+            tail[s][k] = t;
+            if (t) wl[pos[s][k]] = u;
+            count += __popcll(b);
+        }
+    }
+    __syncthreads();
+    for (int r = 0; r < count; r += 64) {
+        const int i = r + lane;
+        if (i < count) wl[i] = moro_tail(wl[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < MAXM; ++s)
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+            if (tail[s][k]) z[s][k] = wl[pos[s][k]];
+    __syncthreads();   // the list is rewritten for the next path
+}
+
+// Discounted payoff of a path from its draws z.  F factors; the swaption
+// starts at step m <= MAXM.
 //
 // kAll = true is the reference's arithmetic in full.  kAll = false skips the
 // swap-curve exponentials past the last cash flow: those discount factors
@@ -113,21 +182,13 @@ __device__ __forceinline__ double moro_inv(double u) {
 // the caller recomputes the path with kAll = true when it does not hold.
 template <int F, int MAXM, bool kAll>
 __device__ __forceinline__ double path_payoff(const SwHead& h, const double* __restrict__ q,
-                                              int64_t p, bool* exact) {
+                                              const double (&z)[MAXM][F], bool* exact) {
     const int n = h.n, m = h.m, L = h.L, w = n - 1;
     const int last = kAll ? L : h.last;
     const double* f0 = q;
     const double* mudt = q + n;
     const double* sigma = q + n + w;
     const double* pay = q + n + w + F * w;
-    double z[MAXM][F];
-    const int64_t c0 = h.seed + p * (int64_t)(w * F);
-#pragma unroll
-    for (int s = 0; s < MAXM; ++s) {
-        if (s >= m) break;
-#pragma unroll
-        for (int k = 0; k < F; ++k) z[s][k] = moro_inv(ran_unif(c0 + s * F + k));
-    }
     double df = 1.0, sdf = 1.0, leg = 0.0, grow = 0.0;
     bool finite = true;
     const int dend = min(n, m + L);
@@ -177,14 +238,25 @@ __global__ __launch_bounds__(kSwThreads) void k_sw_paths(const SwHead* __restric
     const int64_t chunk = blockIdx.x % (unsigned)bps;
     const SwHead h = heads[sw];
     const double* q = pool + h.off;
+    constexpr bool kCompact = MAXM * F <= kSwCompactDraws;
+    __shared__ double lds_tail[kCompact ? kSwThreads / 64 : 1][kCompact ? MAXM * F * 64 : 1];
     double sum = 0.0, sumsq = 0.0;
 #pragma unroll 1
     for (int j = 0; j < kSwPer; ++j) {
         const int64_t p = (chunk * kSwPer + j) * kSwThreads + threadIdx.x;
-        if (p < paths) {
+        const bool valid = p < paths;
+        double z[MAXM][F];
+        if constexpr (kCompact) {
+            // every work-item takes part (the draws synchronise the block);
+            // past the last path it draws for the last one and discards them
+            draws_compact<F, MAXM>(h, valid ? p : paths - 1, z, lds_tail[threadIdx.x / 64]);
+        } else {
+            if (valid) draws_direct<F, MAXM>(h, p, z);
+        }
+        if (valid) {
             bool exact;
-            double v = path_payoff<F, MAXM, false>(h, q, p, &exact);
-            if (!exact) v = path_payoff<F, MAXM, true>(h, q, p, &exact);
+            double v = path_payoff<F, MAXM, false>(h, q, z, &exact);
+            if (!exact) v = path_payoff<F, MAXM, true>(h, q, z, &exact);
             sum += v;
             sumsq += v * v;
         }
