@@ -229,8 +229,12 @@ __device__ __forceinline__ double path_payoff(const SwHead& h, const double* __r
 }
 
 // Block b covers swaption b / bps, paths [(b % bps) kSwThreads kSwPer, +kSwThreads kSwPer).
+// The compacted forms hold the draws, their list slots and the payoff state at
+// once; up to 6 draws per path, asking for 6 waves per SIMD keeps them within
+// 80 VGPRs without spilling (the reference's shape: 79).
 template <int F, int MAXM>
-__global__ __launch_bounds__(kSwThreads) void k_sw_paths(const SwHead* __restrict__ heads,
+__global__ __launch_bounds__(kSwThreads)
+__attribute__((amdgpu_waves_per_eu(MAXM * F <= 6 ? 6 : 1))) void k_sw_paths(const SwHead* __restrict__ heads,
                                                          const double* __restrict__ pool, int bps,
                                                          int64_t paths,
                                                          double* __restrict__ partial) {
