@@ -42,6 +42,7 @@
 #include <cstdio>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "bs_internal.hpp"
@@ -428,6 +429,37 @@ int pack_swaption(const bs_swaption_t& s, int64_t i, int64_t seed, SwHead* h,
     return BS_OK;
 }
 
+// The per-call buffers (parameters, partial sums, results) come from one
+// stream-ordered pool per device that keeps freed memory up to 1 GiB: the
+// call ends with a stream synchronisation, at which the default pool would
+// hand its memory back to the device and the next call would allocate anew.
+constexpr int kSwDevices = 64;
+
+hipError_t pool_of(int dev, hipMemPool_t* out) {
+    static std::mutex mu;
+    static hipMemPool_t pools[kSwDevices] = {};
+    if (dev < 0 || dev >= kSwDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pools[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t p = nullptr;
+        hipError_t e = hipMemPoolCreate(&p, &props);
+        if (e != hipSuccess) return e;
+        uint64_t keep = uint64_t(1) << 30;
+        e = hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            (void)hipMemPoolDestroy(p);
+            return e;
+        }
+        pools[dev] = p;
+    }
+    *out = pools[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 }  // namespace bs
 
@@ -473,9 +505,15 @@ extern "C" int bs_swaptions_host_f64(const bs_swaption_t* sw, int64_t n, int64_t
         const size_t pb = (pool.size() * sizeof(double) + 255) & ~(size_t)255;
         const size_t qb = (size_t)blocks * 2 * sizeof(double);
         const size_t ob = (size_t)n * 2 * sizeof(double);
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+        hipMemPool_t mem = nullptr;
+        e = pool_of(dev, &mem);
+        if (e != hipSuccess) return hip_error(e, "hipMemPoolCreate");
         char* base = nullptr;
-        hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&base), hb + pb + qb + ob, st);
-        if (e != hipSuccess) return hip_error(e, "hipMallocAsync");
+        e = hipMallocFromPoolAsync(reinterpret_cast<void**>(&base), hb + pb + qb + ob, mem, st);
+        if (e != hipSuccess) return hip_error(e, "hipMallocFromPoolAsync");
         SwHead* dh = reinterpret_cast<SwHead*>(base);
         double* dp = reinterpret_cast<double*>(base + hb);
         double* dq = reinterpret_cast<double*>(base + hb + pb);
