@@ -40,6 +40,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 
 #include <algorithm>
 #include <mutex>
@@ -519,10 +520,11 @@ extern "C" int bs_swaptions_host_f64(const bs_swaption_t* sw, int64_t n, int64_t
         double* dq = reinterpret_cast<double*>(base + hb + pb);
         double* dout = reinterpret_cast<double*>(base + hb + pb + qb);
         std::vector<double> out((size_t)n * 2);
-        e = hipMemcpyAsync(dh, heads.data(), (size_t)n * sizeof(SwHead), hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(dp, pool.data(), pool.size() * sizeof(double),
-                               hipMemcpyHostToDevice, st);
+        // heads and pool leave the host as one copy, laid out as on the device
+        std::vector<char> staged(hb + pool.size() * sizeof(double));
+        memcpy(staged.data(), heads.data(), (size_t)n * sizeof(SwHead));
+        memcpy(staged.data() + hb, pool.data(), pool.size() * sizeof(double));
+        e = hipMemcpyAsync(base, staged.data(), staged.size(), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
             kLaunchByFactors[F - 1](mmax, blocks, st, dh, dp, bps, paths, dq);
             hipLaunchKernelGGL(k_sw_final, dim3((unsigned)n), dim3(kSwThreads), 0, st, dq, bps,

@@ -87,14 +87,22 @@ def _records(sws):
     return recs, keep
 
 
+class Prepared:
+    """The C records of a list of swaptions, built once (the reference builds
+    its parm array before its timed region, HJM_Securities.cpp:266-302)."""
+
+    def __init__(self, sws):
+        self.n = len(sws)
+        self.recs, self._keep = _records(sws)
+
+
 def price(sws, trials: int = DEFAULT_TRIALS, seed: int = 0, block_size: int = BLOCK_SIZE,
           stream=None):
-    """(price[n], stderr[n]) of swaptions sws, swaption i with seed + i."""
-    n = len(sws)
-    p = np.empty(n)
-    e = np.empty(n)
-    recs, keep = _records(sws)
-    capi.call("bs_swaptions_host_f64", recs, n, trials, seed, block_size,
+    """(price[n], stderr[n]) of swaptions sws (a list of dicts or a Prepared),
+    swaption i with seed + i."""
+    prep = sws if isinstance(sws, Prepared) else Prepared(sws)
+    p = np.empty(prep.n)
+    e = np.empty(prep.n)
+    capi.call("bs_swaptions_host_f64", prep.recs, prep.n, trials, seed, block_size,
               p.ctypes.data, e.ctypes.data, stream)
-    del keep
     return p, e
