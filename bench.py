@@ -291,7 +291,7 @@ SW_TRIALS = 1_000_000    # PARSEC's native run: 128 swaptions, 1,000,000 trials 
 SW_COUNT = 128
 
 
-def swaptions_bench(no_cpu: bool, calls: int = 3) -> dict:
+def swaptions_bench(no_cpu: bool, calls: int = 5) -> dict:
     """The swaptions map on PARSEC's native input: every path of every
     swaption on the GPU (bs_swaptions_host_f64), timed per whole call (host
     records in, prices out; the records built once before, as the reference
@@ -304,6 +304,8 @@ def swaptions_bench(no_cpu: bool, calls: int = 3) -> dict:
     W.price(sws[:4], 10_000, base)            # first call: code object load
     prep = W.Prepared(sws)                    # the records, built once as the driver does
     st, e0, e1 = capi.Stream(), capi.Event(), capi.Event()
+    for _ in range(6):                        # clocks settle over ~20 ms of load
+        W.price(prep, SW_TRIALS, base, stream=st.handle)
     wall, gpu = [], []
     for _ in range(calls):
         t0 = time.perf_counter()
@@ -316,6 +318,7 @@ def swaptions_bench(no_cpu: bool, calls: int = 3) -> dict:
     out = {"workload": f"{SW_COUNT} swaptions x {SW_TRIALS} trials (PARSEC native input): "
                        "HJM, 11 steps, 3 factors, FP64",
            "seconds_per_call": min(wall), "gpu_ms_per_call": min(gpu),
+           "median_seconds_per_call": float(np.median(wall)), "timed_calls": calls,
            "paths_per_s": paths / min(wall), "swaptions_per_s": SW_COUNT / min(wall),
            "price_min": float(p.min()), "price_max": float(p.max())}
     st.destroy()
