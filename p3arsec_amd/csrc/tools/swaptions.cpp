@@ -35,7 +35,9 @@ constexpr double kFactorVols[kFactors][kSteps - 1] = {
      -.001250}};
 
 // The driver's generator on counter *s (advanced by one): Park-Miller's
-// minimal standard generator of 1513517 c mod (2^31 - 1), scaled.
+// minimal standard generator of 1513517 c mod (2^31 - 1), scaled.  For a
+// negative counter both remainders are <= 0; the sign correction leaves the
+// same residue in [0, 2^31 - 1) as the reference's Schrage steps.
 double ran_unif(long* s) {
     const long long m = 2147483647LL;
     long long x = *s;
@@ -87,10 +89,6 @@ int main(int argc, char** argv) {
            nthreads, nswaptions);
     if (nthreads < 1 || nthreads > kMaxThreads) {
         fprintf(stderr, "Number of threads must be between 1 and %d.\n", kMaxThreads);
-        return 1;
-    }
-    if (seed < 0) {
-        fprintf(stderr, "Error: the seed must be >= 0\n");
         return 1;
     }
     const long base = (long)(2147483647L * ran_unif(&seed));

@@ -38,9 +38,11 @@ _M31 = 2147483647
 
 
 def ran_unif(seed: list) -> float:
-    """One draw of the reference's generator from counter seed[0] >= 0
-    (advanced by one): MINSTD(1513517 c mod (2^31-1)) * 4.656612875e-10, for
-    the driver's parameter draws on the host."""
+    """One draw of the reference's generator from counter seed[0] (advanced by
+    one): MINSTD(1513517 c mod (2^31-1)) * 4.656612875e-10, for the driver's
+    parameter draws on the host.  The reference's C remainders and sign
+    correction end on the residue in [0, 2^31-1), which Python's % gives
+    directly, negative counters included."""
     c = seed[0]
     seed[0] = c + 1
     x = (16807 * ((c * 1513517) % _M31)) % _M31
@@ -50,8 +52,6 @@ def ran_unif(seed: list) -> float:
 def parsec_portfolio(n: int, seed: int = DEFAULT_SEED):
     """(swaptions, base seed): the driver's n swaptions; swaption i is priced
     with base seed + i (HJM_Securities.cpp:198,275-296)."""
-    if seed < 0:
-        raise ValueError("seed < 0")
     s = [seed]
     base = int(_M31 * ran_unif(s))
     # the driver adds .005 step by step: keep its sums exactly

@@ -53,7 +53,7 @@ def test_schrage_equals_direct_modular_arithmetic():
 
 
 def test_host_generator_matches_oracle():
-    for seed in (1979, 0, 7, 2 ** 31 + 5):
+    for seed in (1979, 0, 7, 2 ** 31 + 5, -1, -1979, -(2 ** 31) - 3):
         sws, base = W.parsec_portfolio(40, seed)
         b2, years, strike, _ = S.portfolio(40, seed)
         assert base == b2

@@ -38,7 +38,7 @@ def close(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed", [1979, 12345])
+@pytest.mark.parametrize("seed", [1979, 12345, -77])
 def test_cli_matches_oracle(gpu, seed):
     import pyswaptions as S
     from p3arsec_amd import swaptions as W
