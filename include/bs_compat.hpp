@@ -89,6 +89,41 @@ inline int HJM_Swaption_Blocking(double* pdSwaptionPrice, double dStrike, double
     return 1;
 }
 
+// The whole loop over the driver's parm array in one call: the FastFlow
+// ParallelFor (HJM_Securities.cpp:311-323), the pthreads worker (:102-135),
+// the TBB Worker (:74-97) and the SkePU Map<1> (HJM_Securities_skepu.cpp:
+// 48-61, 189-200) each call HJM_Swaption_Blocking for swaption i with seed
+// swaption_seed + i and store dSimSwaptionMeanPrice / dSimSwaptionStdError.
+// Parm is the caller's own record: any type with the reference's field names
+// (dStrike, dCompounding, dMaturity, dTenor, dPaymentInterval, iN, iFactors,
+// dYears, pdYield, ppdFactors, dSimSwaptionMeanPrice, dSimSwaptionStdError).
+// Every path of every swaption runs on the GPU; a record the reference cannot
+// price throws.
+template <class Parm>
+inline void price_swaptions(Parm* swaptions, int nSwaptions, long swaption_seed, long lTrials,
+                            int BLOCKSIZE) {
+    std::vector<bs_swaption_t> rec((size_t)(nSwaptions > 0 ? nSwaptions : 0));
+    std::vector<std::vector<double>> fac(rec.size());
+    for (size_t i = 0; i < rec.size(); ++i) {
+        const Parm& p = swaptions[i];
+        const int w = p.iN > 1 ? p.iN - 1 : 0;
+        fac[i].resize((size_t)p.iFactors * w);
+        for (int k = 0; k < p.iFactors; ++k)
+            for (int j = 0; j < w; ++j) fac[i][(size_t)k * w + j] = p.ppdFactors[k][j];
+        rec[i] = bs_swaption_t{p.dStrike, p.dCompounding, p.dMaturity, p.dTenor,
+                               p.dPaymentInterval, p.dYears, p.iN, p.iFactors, p.pdYield,
+                               fac[i].data()};
+    }
+    std::vector<double> price(rec.size()), err(rec.size());
+    check(bs_swaptions_host_f64(rec.data(), (int64_t)rec.size(), lTrials, swaption_seed,
+                                BLOCKSIZE, price.data(), err.data(), nullptr),
+          "price_swaptions");
+    for (size_t i = 0; i < rec.size(); ++i) {
+        swaptions[i].dSimSwaptionMeanPrice = price[i];
+        swaptions[i].dSimSwaptionStdError = err[i];
+    }
+}
+
 // bs_thread drop-in over the caller's globals (blackscholes.c:102-113).
 inline void bind_globals(int numOptions, int nThreads, float* sptprice, float* strike,
                          float* rate, float* volatility, float* otime, int* otype,

@@ -75,3 +75,22 @@ def test_compat_hjm_swaption_blocking(gpu):
         assert (math.isnan(se) and math.isnan(oe)) or abs(se - oe) <= 1e-8 * oe + 1e-15, (se, oe)
     p = subprocess.run([DEMO, "5.5", "0.1", "-1", "100"], capture_output=True, text=True)
     assert p.returncode == 1 and "seed" in p.stderr       # refused, as an exception
+
+
+@pytest.mark.gpu
+def test_compat_price_swaptions_over_parm_records(gpu):
+    """bs_compat::price_swaptions over a record with the reference's parm field
+    names: the whole driver loop in one call, each swaption as the oracle."""
+    import pyswaptions as S
+    from p3arsec_amd import swaptions as W
+    p = subprocess.run([DEMO, "map", "24", "5000"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rows = [line.split() for line in p.stdout.strip().splitlines()]
+    sws, base = W.parsec_portfolio(24)
+    assert len(rows) == 24
+    for i, price, se in rows:
+        i = int(i)
+        op, oe = S.price_blocking(sws[i], base + i, 5000)
+        assert close(float(price), op), (i, price, op)
+        se = float(se)
+        assert (math.isnan(se) and math.isnan(oe)) or abs(se - oe) <= 1e-8 * oe + 1e-15, (se, oe)
