@@ -415,7 +415,8 @@ def main() -> int:
     ARGS = args
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the map needs no collectives; sharding is described in DESIGN.md)",
+              "(the map needs no collectives: each GPU would price a contiguous range "
+              "of the book on its own)",
               file=sys.stderr)
         return 2
 
