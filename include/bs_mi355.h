@@ -201,7 +201,10 @@ typedef struct {
  * n_factors below 2^42, the swap's start step x n_factors <= 64; a record
  * the reference would index out of its arrays with (maturity past the
  * horizon, tenor past the curve, payment interval under one step) is
- * BS_ERR_INVALID. */
+ * BS_ERR_INVALID.  A call's device buffers (about 1.5 KB per swaption plus
+ * 16 bytes per 1024 paths) come from a stream-ordered pool the library keeps
+ * per device, which holds up to 1 GiB of freed memory for later calls until
+ * the process ends.  Calls on different threads and streams are independent. */
 int bs_swaptions_host_f64(const bs_swaption_t *sw, int64_t n, int64_t trials, int64_t seed,
                           int32_t block_size, double *price, double *std_error,
                           bs_stream_t stream);
