@@ -1,8 +1,10 @@
 /*
- * bs_mi355.h -- C ABI of libbs_mi355.so, the MI355X (gfx950) Black-Scholes map.
+ * bs_mi355.h -- C ABI of libbs_mi355.so, the MI355X (gfx950) Black-Scholes map
+ * (and, beside it, the suite's swaptions map).
  *
  * Reference: ParaGroup/p3arsec, PARSEC blackscholes
- *   (parsec-ff/pkgs/apps/blackscholes/src/blackscholes.c, "BS" below).
+ *   (parsec-ff/pkgs/apps/blackscholes/src/blackscholes.c, "BS" below) and
+ *   swaptions (parsec-ff/pkgs/apps/swaptions/src, "SW" below).
  * The reference compiles its pricing path into one C++ translation unit with
  * no exported header; each entry point below names the interface it replaces.
  *
