@@ -154,3 +154,19 @@ def test_native_workload(gpu):
     for i in (0, 127):
         check((p[i], e[i]), S.price_chain(sws[i], base + i, 1_000_000), i)
     print(f"128 x 1M: prices {p.min():.4f} .. {p.max():.4f}")
+
+
+def test_counters_near_the_limit(gpu):
+    """Seeds whose last counters come within a few thousand of 2^42, where
+    1513517 c approaches 2^63: the kernel's Mersenne reduction against the
+    oracle's literal 64-bit remainders."""
+    s = sw(years=5.5, strike=0.13)
+    trials = 1000                      # 1008 paths x 30 counters each
+    for slack in (5, 1000, 123457):
+        seed = 2 ** 42 - 1008 * 30 - slack
+        got = W.price([s], trials, seed)
+        check((got[0][0], got[1][0]), S.price_blocking(s, seed, trials), slack)
+    # one past the limit is refused
+    from p3arsec_amd import capi
+    with pytest.raises(capi.BSError):
+        W.price([s], trials, 2 ** 42 - 1008 * 30 + 1)
