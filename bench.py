@@ -321,6 +321,18 @@ def swaptions_bench(no_cpu: bool, calls: int = 5) -> dict:
            "median_seconds_per_call": float(np.median(wall)), "timed_calls": calls,
            "paths_per_s": paths / min(wall), "swaptions_per_s": SW_COUNT / min(wall),
            "price_min": float(p.min()), "price_max": float(p.max())}
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)["swaptions"]
+    except (OSError, KeyError, ValueError):
+        prof = {}
+    if prof.get("paths") == paths:
+        # the paths kernel is bound by FP64 VALU issue: its committed
+        # instruction count's issue time against the whole call's HIP-event time
+        floor_ms = prof["valu_issue_us"] / 1e3
+        out["valu"] = {"issue_floor_ms": floor_ms, "frac_of_call": floor_ms / min(gpu),
+                       "source": "profiles/hbm_traffic.json (rocprofv3 --pmc SQ_INSTS_VALU)"}
     st.destroy()
     e0.destroy()
     e1.destroy()
