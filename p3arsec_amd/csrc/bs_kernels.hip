@@ -68,17 +68,20 @@ Op make_op(const CndfArgs<double>& a) { return Op{a.x, a.out}; }
 // issues the loads of a work-item's next group before pricing the current one
 // (register double buffering), so a wave overlaps its own HBM latency with its
 // arithmetic; it holds one more group of registers, so its occupancy target
-// is two waves lower.
-template <class Op, bool NT, bool PIPE>
+// is two waves lower.  Ix is the group index type: uint32_t when every array
+// spans less than 4 GiB (32-bit offsets on the arrays' SGPR bases; ngroups <
+// 2^30 there, so g + stride cannot wrap), int64_t otherwise.
+template <class Op, bool NT, bool PIPE, class Ix>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, PIPE>())))
 void k_stream(Op op, int64_t ngroups, int64_t tail_begin, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t g = gid;
+    const Ix stride = (Ix)gridDim.x * blockDim.x;
+    const Ix gid = (Ix)blockIdx.x * blockDim.x + threadIdx.x;
+    const Ix ng = (Ix)ngroups;
+    Ix g = gid;
     if constexpr (PIPE) {
-        if (g < ngroups) {
+        if (g < ng) {
             typename Op::Group cur = op.template load<NT>(g);
-            for (int64_t gn = g + stride; gn < ngroups; gn += stride) {
+            for (Ix gn = g + stride; gn < ng; gn += stride) {
                 const typename Op::Group nxt = op.template load<NT>(gn);
                 op.template finish<NT>(g, cur);
                 cur = nxt;
@@ -87,12 +90,12 @@ void k_stream(Op op, int64_t ngroups, int64_t tail_begin, int64_t n) {
             op.template finish<NT>(g, cur);
         }
     } else {
-        for (; g < ngroups; g += stride) {
+        for (; g < ng; g += stride) {
             const typename Op::Group x = op.template load<NT>(g);
             op.template finish<NT>(g, x);
         }
     }
-    if (gid < n - tail_begin) op.scalar(tail_begin + gid);
+    if ((int64_t)gid < n - tail_begin) op.scalar(tail_begin + (int64_t)gid);
 }
 
 // One option per work-item, any alignment.
@@ -123,8 +126,12 @@ hipError_t launch_form(const Op& op, int64_t n, hipStream_t st) {
     const int64_t cap = mb > 0 ? mb : Op::kBlocks;
     const int nt = bt > 0 ? bt : Op::kThreadsPref;
     const unsigned grid = grid_for(std::max<int64_t>(ngroups, 1), cap, nt);
-    hipLaunchKernelGGL((k_stream<Op, NT, PIPE>), dim3(grid), dim3(nt), 0, st, op, ngroups,
-                       ngroups * VW, n);
+    if (n < (int64_t(1) << 32) / Op::kElemBytes)
+        hipLaunchKernelGGL((k_stream<Op, NT, PIPE, uint32_t>), dim3(grid), dim3(nt), 0, st, op,
+                           ngroups, ngroups * VW, n);
+    else
+        hipLaunchKernelGGL((k_stream<Op, NT, PIPE, int64_t>), dim3(grid), dim3(nt), 0, st, op,
+                           ngroups, ngroups * VW, n);
     return hipGetLastError();
 }
 

@@ -73,18 +73,22 @@ __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(
 // number (b = 0 or +-inf, a = +-inf) the unrefined quotient already is IEEE's
 // answer (+-inf, 0 or NaN), so it is returned: an option with T = 0 or
 // sigma = 0 divides by zero in the reference too and prices from d = +-inf.
+template <bool kChecked = true>
 __device__ __forceinline__ float div_rn(float a, float b) {
     const float r = hw_rcp(b);
     const float q = a * r;
     const float e = __builtin_fmaf(-b, q, a);
     const float x = __builtin_fmaf(e, r, q);
+    if constexpr (!kChecked) return x;
     return x == x ? x : q;
 }
 
+template <bool kChecked = true>
 __device__ __forceinline__ float recip_rn(float b) {
     const float r = hw_rcp(b);
     const float e = __builtin_fmaf(-b, r, 1.0f);
     const float x = __builtin_fmaf(e, r, r);
+    if constexpr (!kChecked) return x;
     return x == x ? x : r;
 }
 
@@ -130,11 +134,28 @@ __device__ __forceinline__ float cndf_grad(float ax, float k, float poly, float 
 
 // ---------------------------------------------------------------------------
 // Ref32: the reference's float arithmetic.
-struct Ref32 {
+//
+// Ref32Arith<true> carries the special-value handling: the NaN tests of the
+// refined quotient and reciprocal, and CNDF's cap on |x|.  Ref32Arith<false>
+// leaves them out (12 fewer VALU instructions per option).  The two give the
+// same bits for every option whose unchecked price is not NaN:
+//   - a refinement's NaN test changes a result only where the refined value is
+//     NaN; a NaN quotient S/K or d1 reaches both CNDF values, a NaN reciprocal
+//     reaches its CNDF value, and a NaN CNDF value makes the price NaN for both
+//     option types (S * NaN and 1 - NaN are NaN);
+//   - the cap changes nothing for finite |x| > 1e10: both forms give n = 0
+//     (e^{-x^2/2} underflows, and -x^2/2 = -inf gives 0), a finite k, a tail
+//     of 0 and CNDF = 1 exactly, and a gradient n * (...) = +0; only |x| = inf
+//     differs, where the unchecked product inf * c_hi + inf * c_lo is NaN.
+// So bs_price / bs_greeks run Ref32::Unchecked and recompute with Ref32 only
+// the options whose price came out NaN (never in a book of finite positive
+// inputs: the branch costs one compare per option).
+template <bool kChecked>
+struct Ref32Arith {
     using T = float;
     __device__ static __forceinline__ float exp(float x) { return exp_hw(x); }
     __device__ static __forceinline__ float log(float x) { return log_hw(x); }
-    __device__ static __forceinline__ float div(float a, float b) { return div_rn(a, b); }
+    __device__ static __forceinline__ float div(float a, float b) { return div_rn<kChecked>(a, b); }
     __device__ static __forceinline__ float sqrt(float x) { return sqrt_rn(x); }
 
     // blackscholes.c:126-184, statement by statement.
@@ -150,12 +171,12 @@ struct Ref32 {
         // the capped value; uncapped, |x| = inf would meet inf*hi + inf*lo = NaN
         // in mul_dc where the reference's double product is inf.  NaN passes.
         float ax = __builtin_fabsf(x);   // |-0| = +0 gives the same bits below
-        ax = ax > 1e10f ? 1e10f : ax;
+        if constexpr (kChecked) ax = ax > 1e10f ? 1e10f : ax;
         float n = exp(-0.5f * ax * ax);
         n = mul_dc(n, c0);
         float k = mul_dc(ax, cp);
         k = 1.0f + k;
-        k = recip_rn(k);
+        k = recip_rn<kChecked>(k);
         const float k2 = k * k;
         const float k3 = k2 * k;
         const float k4 = k3 * k;
@@ -173,10 +194,15 @@ struct Ref32 {
     }
 };
 
+struct Ref32 : Ref32Arith<true> {
+    using Unchecked = Ref32Arith<false>;
+};
+
 // ---------------------------------------------------------------------------
 // Fast32: hardware transcendentals.
 struct Fast32 {
     using T = float;
+    using Unchecked = Fast32;
     __device__ static __forceinline__ float exp(float x) {
         return __builtin_amdgcn_exp2f(x * kLog2e);
     }
@@ -273,6 +299,7 @@ __device__ __forceinline__ double log64(double x) {
 // with fptype = double, prices differ by a few units in the last place.
 struct Ref64 {
     using T = double;
+    using Unchecked = Ref64;
     __device__ static __forceinline__ double exp(double x) { return ::exp(x); }
     __device__ static __forceinline__ double log(double x) { return log64(x); }
     __device__ static __forceinline__ double div(double a, double b) { return a / b; }
@@ -341,11 +368,25 @@ __device__ __forceinline__ T bs_combine(T s, const Forward<T>& f, bool put) {
     return f.fv * m2 - s * m1;
 }
 
+// A policy with a distinct unchecked form prices with it first and recomputes
+// with the checked one where the price is NaN (Ref32: identical bits).
+template <class M>
+constexpr bool has_unchecked() {
+    return !__is_same(typename M::Unchecked, M);
+}
+
 template <class M>
 __device__ __forceinline__ typename M::T bs_price(typename M::T s, typename M::T k,
                                                   typename M::T r, typename M::T v,
                                                   typename M::T t, bool put) {
-    return bs_combine(s, bs_forward<M, false>(s, k, r, v, t), put);
+    if constexpr (has_unchecked<M>()) {
+        using U = typename M::Unchecked;
+        typename M::T p = bs_combine(s, bs_forward<U, false>(s, k, r, v, t), put);
+        if (__builtin_expect(p != p, 0)) p = bs_combine(s, bs_forward<M, false>(s, k, r, v, t), put);
+        return p;
+    } else {
+        return bs_combine(s, bs_forward<M, false>(s, k, r, v, t), put);
+    }
 }
 
 struct Greeks32 {
@@ -360,8 +401,8 @@ struct Greeks32 {
 // Outputs: delta = dP/dS, vega = dP/dsigma, rho = dP/dr, theta = -dP/dT (per
 // year); vega and rho are per unit change, not per percent.
 template <class M>
-__device__ __forceinline__ Greeks32 bs_greeks(float s, float k, float r, float v, float t,
-                                              bool put) {
+__device__ __forceinline__ Greeks32 bs_greeks_of(float s, float k, float r, float v, float t,
+                                                 bool put) {
 #pragma clang fp contract(off)
     const Forward<float> f = bs_forward<M, true>(s, k, r, v, t);
     Greeks32 o;
@@ -382,6 +423,21 @@ __device__ __forceinline__ Greeks32 bs_greeks(float s, float k, float r, float v
                                        __builtin_fmaf(0.5f * sqrt_t_bar, v * rden, -(fv_dr * r)));
     o.theta = -t_bar;
     return o;
+}
+
+// Unchecked first, checked where the price is NaN: every checked-only
+// difference in the forward pass makes the price NaN (see Ref32Arith), and
+// the adjoint reads nothing else.
+template <class M>
+__device__ __forceinline__ Greeks32 bs_greeks(float s, float k, float r, float v, float t,
+                                              bool put) {
+    if constexpr (has_unchecked<M>()) {
+        Greeks32 o = bs_greeks_of<typename M::Unchecked>(s, k, r, v, t, put);
+        if (__builtin_expect(o.price != o.price, 0)) o = bs_greeks_of<M>(s, k, r, v, t, put);
+        return o;
+    } else {
+        return bs_greeks_of<M>(s, k, r, v, t, put);
+    }
 }
 
 }  // namespace bs

@@ -3,8 +3,9 @@
 // (bs_serve.hip), so both compute every option with the same code.
 //
 // Each op: Group (registers of one VW-option group), load<NT>(g),
-// finish<NT>(g, x) (compute + store), scalar(i) for options outside whole
-// groups, kWaves (the occupancy the register allocator is held to: 8 waves
+// finish<NT>(g, x) (compute + store; g is the group index, int64_t or, for
+// arrays under 4 GiB, uint32_t), scalar(i) for options outside whole groups,
+// kElemBytes (the widest array element, for the 4 GiB test), kWaves (the occupancy the register allocator is held to: 8 waves
 // per SIMD = 64 VGPRs), kBlocks (default grid cap, 0 = one group per
 // work-item) and kPipe (default loop form); the defaults and each launcher's
 // preferred width are the fastest forms measured on MI355X for 10M options.
@@ -40,6 +41,21 @@ __device__ __forceinline__ void st(T* p, T v) {
     else *p = v;
 }
 
+// Element g of an array.  I = int64_t: 64-bit index arithmetic.  I = uint32_t
+// (the launcher guarantees every array of the call spans less than 4 GiB): a
+// 32-bit byte offset added to the array's base, which the compiler folds into
+// the global load/store's SGPR-base + VGPR-offset form, so one offset register
+// serves every array of the same element size.
+template <class T, class I>
+__device__ __forceinline__ T* at(T* base, I g) {
+    if constexpr (__is_same(I, uint32_t)) {
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + (uint32_t)(g * (uint32_t)sizeof(T)));
+    } else {
+        return base + g;
+    }
+}
+
 __device__ __forceinline__ bool put_of(uint32_t word, int j) {
     return ((word >> (8 * j)) & 0xffu) != 0u;
 }
@@ -47,6 +63,7 @@ __device__ __forceinline__ bool put_of(uint32_t word, int j) {
 template <class Math, int VW>
 struct PriceF32 {
     static constexpr int kVW = VW;
+    static constexpr int kElemBytes = 4;   // widest array element
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
@@ -59,24 +76,24 @@ struct PriceF32 {
         F s, k, r, v, t;
         tword<VW> ty;
     };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ Group load(Ix g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const tword<VW>*>(type) + g);
+        x.s = ld<NT>(at(reinterpret_cast<const F*>(in.s), g));
+        x.k = ld<NT>(at(reinterpret_cast<const F*>(in.k), g));
+        x.r = ld<NT>(at(reinterpret_cast<const F*>(in.r), g));
+        x.v = ld<NT>(at(reinterpret_cast<const F*>(in.v), g));
+        x.t = ld<NT>(at(reinterpret_cast<const F*>(in.t), g));
+        x.ty = ld<NT>(at(reinterpret_cast<const tword<VW>*>(type), g));
         return x;
     }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ void finish(Ix g, const Group& x) const {
         F p;
 #pragma unroll
         for (int j = 0; j < VW; ++j)
             p[j] = bs_price<Math>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
-        st<NT>(reinterpret_cast<F*>(out) + g, p);
+        st<NT>(at(reinterpret_cast<F*>(out), g), p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = bs_price<Math>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
@@ -86,6 +103,7 @@ struct PriceF32 {
 template <int VW>
 struct PriceI32F32 {
     static constexpr int kVW = VW;
+    static constexpr int kElemBytes = 4;   // widest array element
     static constexpr int kWaves = 7;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
@@ -99,24 +117,24 @@ struct PriceI32F32 {
         F s, k, r, v, t;
         I ty;
     };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ Group load(Ix g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const I*>(type) + g);
+        x.s = ld<NT>(at(reinterpret_cast<const F*>(in.s), g));
+        x.k = ld<NT>(at(reinterpret_cast<const F*>(in.k), g));
+        x.r = ld<NT>(at(reinterpret_cast<const F*>(in.r), g));
+        x.v = ld<NT>(at(reinterpret_cast<const F*>(in.v), g));
+        x.t = ld<NT>(at(reinterpret_cast<const F*>(in.t), g));
+        x.ty = ld<NT>(at(reinterpret_cast<const I*>(type), g));
         return x;
     }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ void finish(Ix g, const Group& x) const {
         F p;
 #pragma unroll
         for (int j = 0; j < VW; ++j)
             p[j] = bs_price<Ref32>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], x.ty[j] != 0);
-        st<NT>(reinterpret_cast<F*>(out) + g, p);
+        st<NT>(at(reinterpret_cast<F*>(out), g), p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = bs_price<Ref32>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
@@ -127,6 +145,7 @@ struct PriceI32F32 {
 template <int VW>
 struct PriceF64 {
     static constexpr int kVW = 2;
+    static constexpr int kElemBytes = 8;   // widest array element
     static constexpr int kWaves = 5;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
@@ -139,24 +158,24 @@ struct PriceF64 {
         D s, k, r, v, t;
         uint16_t ty;
     };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ Group load(Ix g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const D*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const D*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const D*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const D*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const D*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const uint16_t*>(type) + g);
+        x.s = ld<NT>(at(reinterpret_cast<const D*>(in.s), g));
+        x.k = ld<NT>(at(reinterpret_cast<const D*>(in.k), g));
+        x.r = ld<NT>(at(reinterpret_cast<const D*>(in.r), g));
+        x.v = ld<NT>(at(reinterpret_cast<const D*>(in.v), g));
+        x.t = ld<NT>(at(reinterpret_cast<const D*>(in.t), g));
+        x.ty = ld<NT>(at(reinterpret_cast<const uint16_t*>(type), g));
         return x;
     }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ void finish(Ix g, const Group& x) const {
         D p;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
             p[j] = bs_price<Ref64>(x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j));
-        st<NT>(reinterpret_cast<D*>(out) + g, p);
+        st<NT>(at(reinterpret_cast<D*>(out), g), p);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = bs_price<Ref64>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i], type[i] != 0);
@@ -166,6 +185,7 @@ struct PriceF64 {
 template <int VW>
 struct GreeksF32 {
     static constexpr int kVW = VW;
+    static constexpr int kElemBytes = 4;   // widest array element
     static constexpr int kWaves = VW == 4 ? 5 : 8;
     static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
@@ -178,19 +198,19 @@ struct GreeksF32 {
         F s, k, r, v, t;
         tword<VW> ty;
     };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ Group load(Ix g) const {
         Group x;
-        x.s = ld<NT>(reinterpret_cast<const F*>(in.s) + g);
-        x.k = ld<NT>(reinterpret_cast<const F*>(in.k) + g);
-        x.r = ld<NT>(reinterpret_cast<const F*>(in.r) + g);
-        x.v = ld<NT>(reinterpret_cast<const F*>(in.v) + g);
-        x.t = ld<NT>(reinterpret_cast<const F*>(in.t) + g);
-        x.ty = ld<NT>(reinterpret_cast<const tword<VW>*>(type) + g);
+        x.s = ld<NT>(at(reinterpret_cast<const F*>(in.s), g));
+        x.k = ld<NT>(at(reinterpret_cast<const F*>(in.k), g));
+        x.r = ld<NT>(at(reinterpret_cast<const F*>(in.r), g));
+        x.v = ld<NT>(at(reinterpret_cast<const F*>(in.v), g));
+        x.t = ld<NT>(at(reinterpret_cast<const F*>(in.t), g));
+        x.ty = ld<NT>(at(reinterpret_cast<const tword<VW>*>(type), g));
         return x;
     }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& x) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ void finish(Ix g, const Group& x) const {
         F p, d, ve, rh, th;
 #pragma unroll
         for (int j = 0; j < VW; ++j) {
@@ -202,11 +222,11 @@ struct GreeksF32 {
             rh[j] = o.rho;
             th[j] = o.theta;
         }
-        st<NT>(reinterpret_cast<F*>(out.price) + g, p);
-        st<NT>(reinterpret_cast<F*>(out.delta) + g, d);
-        st<NT>(reinterpret_cast<F*>(out.vega) + g, ve);
-        st<NT>(reinterpret_cast<F*>(out.rho) + g, rh);
-        st<NT>(reinterpret_cast<F*>(out.theta) + g, th);
+        st<NT>(at(reinterpret_cast<F*>(out.price), g), p);
+        st<NT>(at(reinterpret_cast<F*>(out.delta), g), d);
+        st<NT>(at(reinterpret_cast<F*>(out.vega), g), ve);
+        st<NT>(at(reinterpret_cast<F*>(out.rho), g), rh);
+        st<NT>(at(reinterpret_cast<F*>(out.theta), g), th);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         const Greeks32 o = bs_greeks<Ref32>(in.s[i], in.k[i], in.r[i], in.v[i], in.t[i],
@@ -222,6 +242,7 @@ struct GreeksF32 {
 template <int VW>
 struct CndfF32 {
     static constexpr int kVW = VW;
+    static constexpr int kElemBytes = 4;   // widest array element
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
@@ -232,16 +253,16 @@ struct CndfF32 {
     struct Group {
         F x;
     };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        return Group{ld<NT>(reinterpret_cast<const F*>(x) + g)};
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ Group load(Ix g) const {
+        return Group{ld<NT>(at(reinterpret_cast<const F*>(x), g))};
     }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ void finish(Ix g, const Group& a) const {
         F y;
 #pragma unroll
         for (int j = 0; j < VW; ++j) y[j] = Ref32::cndf<false>(a.x[j], nullptr);
-        st<NT>(reinterpret_cast<F*>(out) + g, y);
+        st<NT>(at(reinterpret_cast<F*>(out), g), y);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = Ref32::cndf<false>(x[i], nullptr);
@@ -251,6 +272,7 @@ struct CndfF32 {
 template <int VW>
 struct CndfF64 {
     static constexpr int kVW = 2;
+    static constexpr int kElemBytes = 8;   // widest array element
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
@@ -261,16 +283,16 @@ struct CndfF64 {
     struct Group {
         D x;
     };
-    template <bool NT>
-    __device__ __forceinline__ Group load(int64_t g) const {
-        return Group{ld<NT>(reinterpret_cast<const D*>(x) + g)};
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ Group load(Ix g) const {
+        return Group{ld<NT>(at(reinterpret_cast<const D*>(x), g))};
     }
-    template <bool NT>
-    __device__ __forceinline__ void finish(int64_t g, const Group& a) const {
+    template <bool NT, class Ix = int64_t>
+    __device__ __forceinline__ void finish(Ix g, const Group& a) const {
         D y;
         y[0] = Ref64::cndf<false>(a.x[0], nullptr);
         y[1] = Ref64::cndf<false>(a.x[1], nullptr);
-        st<NT>(reinterpret_cast<D*>(out) + g, y);
+        st<NT>(at(reinterpret_cast<D*>(out), g), y);
     }
     __device__ __forceinline__ void scalar(int64_t i) const {
         out[i] = Ref64::cndf<false>(x[i], nullptr);
