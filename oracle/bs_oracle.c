@@ -44,6 +44,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 /* ------------------------------------------------------------------------ */
 /* Abramowitz & Stegun 26.2.17 constants (Hull's CNDF), as double literals.  */
@@ -239,31 +240,103 @@ void bso_greeks_fd(double s, double k, double r, double v, double t, int otype,
 }
 
 /* ------------------------------------------------------------------------ */
-/* Vector helpers for the Python tests (plain loops; single thread).         */
+/* Vector helpers for the Python tests.  Each element is a pure function of   */
+/* its inputs, so the helpers split [0, n) into contiguous blocks over up to  */
+/* 64 threads (one per online CPU) with results independent of the split.    */
+enum { VEC_CNDF32, VEC_CNDF64, VEC_PRICE64, VEC_EXACT, VEC_GREEKS_EXACT, VEC_GREEKS_FD };
+
+typedef struct {
+    int op;
+    int64_t n, begin, end;
+    const float *x32, *s, *k, *r, *v, *t;
+    const double *x64, *ds, *dk, *dr, *dv, *dt;
+    const uint8_t *type;
+    double h;
+    float *out32;
+    double *out;
+} vec_job;
+
+static void vec_range(const vec_job *j)
+{
+    double g[5];
+    for (int64_t i = j->begin; i < j->end; i++) {
+        switch (j->op) {
+        case VEC_CNDF32: j->out32[i] = bso_cndf_f32(j->x32[i]); break;
+        case VEC_CNDF64: j->out[i] = bso_cndf_f64(j->x64[i]); break;
+        case VEC_PRICE64:
+            j->out[i] = bso_price_f64(j->ds[i], j->dk[i], j->dr[i], j->dv[i], j->dt[i],
+                                      j->type[i] != 0);
+            break;
+        case VEC_EXACT:
+            j->out[i] = bso_price_exact(j->s[i], j->k[i], j->r[i], j->v[i], j->t[i],
+                                        j->type[i] != 0);
+            break;
+        case VEC_GREEKS_EXACT:
+        case VEC_GREEKS_FD:
+            if (j->op == VEC_GREEKS_EXACT)
+                bso_greeks_exact(j->s[i], j->k[i], j->r[i], j->v[i], j->t[i],
+                                 j->type[i] != 0, g);
+            else
+                bso_greeks_fd(j->s[i], j->k[i], j->r[i], j->v[i], j->t[i], j->type[i] != 0,
+                              j->h, g);
+            for (int q = 0; q < 5; q++) j->out[q * j->n + i] = g[q];   /* 5 x n, row per quantity */
+            break;
+        }
+    }
+}
+
+static void *vec_worker(void *arg)
+{
+    vec_range((const vec_job *)arg);
+    return NULL;
+}
+
+static void run_vec(vec_job job)
+{
+    long cpus = sysconf(_SC_NPROCESSORS_ONLN);
+    int64_t nt = cpus < 1 ? 1 : (cpus > 64 ? 64 : cpus);
+    if (nt > job.n / 4096 + 1) nt = job.n / 4096 + 1;   /* small inputs: one thread */
+    vec_job jobs[64];
+    pthread_t tids[64];
+    int started[64] = {0};
+    int64_t chunk = job.n / nt, extra = job.n % nt, start = 0;
+    for (int64_t i = 0; i < nt; i++) {
+        jobs[i] = job;
+        jobs[i].begin = start;
+        jobs[i].end = start + chunk + (i < extra ? 1 : 0);
+        start = jobs[i].end;
+        started[i] = i > 0 && pthread_create(&tids[i], NULL, vec_worker, &jobs[i]) == 0;
+        if (i > 0 && !started[i]) vec_range(&jobs[i]);   /* no thread: run it here */
+    }
+    vec_range(&jobs[0]);
+    for (int64_t i = 1; i < nt; i++)
+        if (started[i]) pthread_join(tids[i], NULL);
+}
+
 void bso_cndf_f32_vec(int64_t n, const float *x, float *out)
 {
-    for (int64_t i = 0; i < n; i++) out[i] = bso_cndf_f32(x[i]);
+    run_vec((vec_job){.op = VEC_CNDF32, .n = n, .x32 = x, .out32 = out});
 }
 
 void bso_cndf_f64_vec(int64_t n, const double *x, double *out)
 {
-    for (int64_t i = 0; i < n; i++) out[i] = bso_cndf_f64(x[i]);
+    run_vec((vec_job){.op = VEC_CNDF64, .n = n, .x64 = x, .out = out});
 }
 
 void bso_price_f64_vec(int64_t n, const double *s, const double *k, const double *r,
                        const double *v, const double *t, const uint8_t *type,
                        double *out)
 {
-    for (int64_t i = 0; i < n; i++)
-        out[i] = bso_price_f64(s[i], k[i], r[i], v[i], t[i], type[i] != 0);
+    run_vec((vec_job){.op = VEC_PRICE64, .n = n, .ds = s, .dk = k, .dr = r, .dv = v, .dt = t,
+                      .type = type, .out = out});
 }
 
 void bso_price_exact_vec(int64_t n, const float *s, const float *k, const float *r,
                          const float *v, const float *t, const uint8_t *type,
                          double *out)
 {
-    for (int64_t i = 0; i < n; i++)
-        out[i] = bso_price_exact(s[i], k[i], r[i], v[i], t[i], type[i] != 0);
+    run_vec((vec_job){.op = VEC_EXACT, .n = n, .s = s, .k = k, .r = r, .v = v, .t = t,
+                      .type = type, .out = out});
 }
 
 /* out is 5 x n (row per quantity). */
@@ -271,22 +344,16 @@ void bso_greeks_exact_vec(int64_t n, const float *s, const float *k, const float
                           const float *v, const float *t, const uint8_t *type,
                           double *out)
 {
-    double g[5];
-    for (int64_t i = 0; i < n; i++) {
-        bso_greeks_exact(s[i], k[i], r[i], v[i], t[i], type[i] != 0, g);
-        for (int j = 0; j < 5; j++) out[j * n + i] = g[j];
-    }
+    run_vec((vec_job){.op = VEC_GREEKS_EXACT, .n = n, .s = s, .k = k, .r = r, .v = v, .t = t,
+                      .type = type, .out = out});
 }
 
 void bso_greeks_fd_vec(int64_t n, const float *s, const float *k, const float *r,
                        const float *v, const float *t, const uint8_t *type, double h,
                        double *out)
 {
-    double g[5];
-    for (int64_t i = 0; i < n; i++) {
-        bso_greeks_fd(s[i], k[i], r[i], v[i], t[i], type[i] != 0, h, g);
-        for (int j = 0; j < 5; j++) out[j * n + i] = g[j];
-    }
+    run_vec((vec_job){.op = VEC_GREEKS_FD, .n = n, .s = s, .k = k, .r = r, .v = v, .t = t,
+                      .type = type, .h = h, .out = out});
 }
 
 /* ------------------------------------------------------------------------ */

@@ -47,8 +47,9 @@ def lib():
         L.swo_chain_payoffs.restype = ctypes.c_int
         L.swo_portfolio.argtypes = [ctypes.c_int, _i64p, _i64p, _f64p, _f64p]
         L.swo_price_batch.argtypes = ([ctypes.c_int] + [_f64p] * 6 + [ctypes.c_int, ctypes.c_int,
-                                      _f64p, _f64p, _i64, _i64, ctypes.c_int, _f64p, _f64p,
-                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int)])
+                                      _f64p, _f64p, _i64, _i64, ctypes.c_int, ctypes.c_int,
+                                      _f64p, _f64p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int)])
         L.swo_price_batch.restype = ctypes.c_double
         _done = True
     return L
@@ -143,9 +144,12 @@ def portfolio(n: int, seed: int = 1979):
     return int(base.value), years, strike, int(s.value)
 
 
-def price_batch(sws, seed: int, trials: int, block: int = 16, nthreads: int = 0):
+def price_batch(sws, seed: int, trials: int, block: int = 16, nthreads: int = 0,
+                chain: bool = False):
     """The FastFlow ParallelFor over swaptions (HJM_Securities.cpp:311-323),
     swaption i with seed + i.  All swaptions share n_steps and n_factors.
+    chain=True prices each swaption by the per-chain form (only the time steps
+    its payoff reads; the same bits as the blocked form).
     Returns (price[ns], stderr[ns], seconds, threads)."""
     ns = len(sws)
     n = len(sws[0]["yield"])
@@ -160,7 +164,8 @@ def price_batch(sws, seed: int, trials: int, block: int = 16, nthreads: int = 0)
     used = ctypes.c_int(0)
     sec = lib().swo_price_batch(ns, col["strike"], col["compounding"], col["maturity"],
                                 col["tenor"], col["interval"], col["years"], n, F, y,
-                                fa.reshape(-1), seed, trials, block, price, se, nthreads,
+                                fa.reshape(-1), seed, trials, block, int(chain), price, se,
+                                nthreads,
                                 ctypes.byref(used))
     if sec < 0:
         raise ValueError("swaption batch failed")

@@ -449,7 +449,7 @@ typedef struct {
     int n, F;
     const double *yield, *factors;
     int64_t seed, trials;
-    int B;
+    int B, chain;
     double *price, *stderr_;
     int ok;
 } swo_block;
@@ -466,11 +466,13 @@ static void *swo_worker(void *arg)
     b->ok = 1;
     for (int i = b->begin; i < b->end; i++) {
         double o[2];
-        if (!swo_swaption_blocking(o, b->strike[i], b->compounding[i], b->maturity[i],
-                                   b->tenor[i], b->interval[i], b->n, b->F, b->years[i],
-                                   b->yield + (size_t)i * b->n,
-                                   b->factors + (size_t)i * b->F * (b->n - 1), b->seed + i,
-                                   b->trials, b->B))
+        int (*price_one)(double *, double, double, double, double, double, int, int, double,
+                         const double *, const double *, int64_t, int64_t, int) =
+            b->chain ? swo_swaption_chain : swo_swaption_blocking;
+        if (!price_one(o, b->strike[i], b->compounding[i], b->maturity[i], b->tenor[i],
+                       b->interval[i], b->n, b->F, b->years[i], b->yield + (size_t)i * b->n,
+                       b->factors + (size_t)i * b->F * (b->n - 1), b->seed + i, b->trials,
+                       b->B))
             b->ok = 0;
         b->price[i] = o[0];
         b->stderr_[i] = o[1];
@@ -479,12 +481,14 @@ static void *swo_worker(void *arg)
 }
 
 /* Prices ns swaptions (yield: ns rows of n; factors: ns blocks of F x (n-1))
-   on nthreads threads (<= 0: every CPU this process may use).  Returns the
-   wall time in seconds, or a negative value on error. */
+   on nthreads threads (<= 0: every CPU this process may use), each by
+   swo_swaption_blocking (chain = 0: every time step, as the reference) or by
+   swo_swaption_chain (chain = 1: only the steps the payoff reads, the same
+   bits).  Returns the wall time in seconds, or a negative value on error. */
 double swo_price_batch(int ns, const double *strike, const double *compounding,
                        const double *maturity, const double *tenor, const double *interval,
                        const double *years, int n, int F, const double *yield,
-                       const double *factors, int64_t seed, int64_t trials, int B,
+                       const double *factors, int64_t seed, int64_t trials, int B, int chain,
                        double *price, double *stderr_, int nthreads, int *used_threads)
 {
     cpu_set_t allowed;
@@ -509,7 +513,7 @@ double swo_price_batch(int ns, const double *strike, const double *compounding,
         const int len = chunk + (i < extra ? 1 : 0);
         blk[i] = (swo_block){start, start + len, cpus[i % ncpu], strike, compounding, maturity,
                              tenor, interval, years, n, F, yield, factors, seed, trials, B,
-                             price, stderr_, 0};
+                             chain, price, stderr_, 0};
         start += len;
         pthread_create(&tid[i], NULL, swo_worker, &blk[i]);
     }

@@ -29,6 +29,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <memory>
 #include <string>
 #include <thread>
@@ -273,6 +274,31 @@ inline bool write_prices(const char* path, const float* price, int n, int thread
     if (fclose(f) != 0) good = false;
     if (!good) *err = std::string("Unable to write to file `") + path + "'.";
     return good;
+}
+
+// The reference's ERR_CHK test (blackscholes.c:333-340 in the FastFlow map,
+// :641-647 in bs_thread, the count printed at :949-951).  The reference runs
+// it inside the NUM_RUNS loop, so every pass tests every option again: the
+// delta DGrefval - price is a float subtraction, fabs of it takes the float
+// overload (the file is compiled as C++), and the comparison promotes that
+// float to double against the double literal 1e-4, so a delta of exactly
+// float(1e-4) = 9.99999974e-5 does not count.  Each failure prints the
+// reference's "Error on ..." line (pass by pass, options in order, as the
+// sequential build prints them) and counts once per pass.  Returns the count.
+inline long long err_chk(const OptionData* data, const float* prices, int n, int passes,
+                         FILE* out) {
+    std::vector<int> bad;
+    for (int i = 0; i < n; ++i) {
+        const float delta = data[i].DGrefval - prices[i];
+        if ((double)std::fabs(delta) >= 1e-4) bad.push_back(i);
+    }
+    for (int j = 0; j < passes && out; ++j)
+        for (int i : bad) {
+            const float delta = data[i].DGrefval - prices[i];
+            fprintf(out, "Error on %d. Computed=%.5f, Ref=%.5f, Delta=%.5f\n", i,
+                    (double)prices[i], (double)data[i].DGrefval, (double)delta);
+        }
+    return (long long)bad.size() * passes;
 }
 
 }  // namespace parsec_io

@@ -6,6 +6,7 @@
 // bytes as one fprintf("%.18f\n") per price.  Exit 0 when every case agrees.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -83,7 +84,32 @@ static std::string records(int n, const char* fmt, const char* sep, uint64_t see
     return s;
 }
 
-int main() {
+// parsec_io_check --errchk <input> <prices> <passes>: runs err_chk on the
+// input's DGrefval column and the prices of a reference output file, printing
+// what the reference's ERR_CHK build prints (its "Error on" lines, then
+// "Num Errors: N").
+static int errchk_mode(const char* input, const char* prices_path, int passes) {
+    std::vector<OptionData> d;
+    std::string err;
+    if (!parsec_io::read_options(input, &d, 1, &err)) {
+        printf("read: %s\n", err.c_str());
+        return 2;
+    }
+    FILE* f = fopen(prices_path, "r");
+    int n = 0;
+    if (!f || fscanf(f, "%i", &n) != 1 || n != (int)d.size()) return 2;
+    std::vector<float> p((size_t)n);
+    for (int i = 0; i < n; ++i)
+        if (fscanf(f, "%f", &p[(size_t)i]) != 1) return 2;
+    fclose(f);
+    const long long k = parsec_io::err_chk(d.data(), p.data(), n, passes, stdout);
+    printf("Num Errors: %lld\n", k);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc == 5 && strcmp(argv[1], "--errchk") == 0)
+        return errchk_mode(argv[2], argv[3], atoi(argv[4]));
     const char* std9 = "%.9g %.9g %.9g 0 %.9g %.9g %c 0 %.9g";
     check_file("regular", records(10007, std9, "\n", 1));
     check_file("native_fixed", records(5003, "%.2f %.2f %.4f 0.00 %.2f %.2f %c 0.00 %.15f", "\n", 2));
@@ -194,5 +220,47 @@ int main() {
         }
     }
     printf("output: %zu prices, byte-identical to fprintf\n", p.size());
+
+    // ERR_CHK (blackscholes.c:333-340): float delta, fabs promoted to double
+    // against 1e-4, counted once per pass.  Expected per case from that rule:
+    // exactly float(1e-4) either way is not an error (it is below the double
+    // 1e-4); the next float up is; 100.0001f - 100 is 9.918e-5 (not an error),
+    // 100.00011f - 100 is 1.068e-4 (an error); a NaN price is not counted.
+    {
+        const float f1e4 = 1e-4f, up = nextafterf(1e-4f, 1.0f);
+        struct Case { float price, ref; bool error; };
+        const Case cases[] = {{0.0f, f1e4, false},     {0.0f, -f1e4, false},
+                              {0.0f, up, true},        {0.0f, -up, true},
+                              {100.0f, 100.0001f, false}, {100.0f, 100.00011f, true},
+                              {NAN, 1.0f, false},      {5.0f, 5.0f, false}};
+        std::vector<OptionData> d;
+        std::vector<float> pr;
+        int want = 0;
+        for (const Case& c : cases) {
+            OptionData o;
+            memset(&o, 0, sizeof(o));
+            o.DGrefval = c.ref;
+            d.push_back(o);
+            pr.push_back(c.price);
+            want += c.error;
+        }
+        const char* path = "/tmp/parsec_io_check_errchk.txt";
+        FILE* f = fopen(path, "w");
+        const long long got = parsec_io::err_chk(d.data(), pr.data(), (int)d.size(), 3, f);
+        fclose(f);
+        std::vector<std::string> lines;
+        char buf[256];
+        f = fopen(path, "r");
+        while (fgets(buf, sizeof(buf), f)) lines.push_back(buf);
+        fclose(f);
+        remove(path);
+        const bool ok = got == 3LL * want && want == 3 && lines.size() == 9 &&
+                        lines[0] == "Error on 2. Computed=0.00000, Ref=0.00010, Delta=0.00010\n" &&
+                        lines[2] == "Error on 5. Computed=100.00000, Ref=100.00011, Delta=0.00011\n" &&
+                        lines[3] == lines[0];
+        printf("err_chk: %lld errors over 3 passes (want %d), %zu lines\n", got, 3 * want,
+               lines.size());
+        if (!ok) ++g_bad;
+    }
     return g_bad ? 1 : 0;
 }

@@ -7,6 +7,8 @@ malformed, empty, hex-counted and special-value inputs, with 1-8 threads
 import os
 import subprocess
 
+import pytest
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 NATIVE = os.path.join(HERE, "native")
 
@@ -17,3 +19,53 @@ def test_threaded_io_matches_fscanf_and_fprintf():
                        timeout=120)
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
+
+
+def test_err_chk_matches_the_compiled_reference(tmp_path):
+    """The CLI's --check (parsec_io::err_chk) against the reference's own
+    ERR_CHK build (oracle/_ref/blackscholes_errchk: blackscholes.c compiled
+    sequentially with -DERR_CHK), on one input whose DGrefval column puts
+    options exactly on, just over and well past the 1e-4 line: both must print
+    the same "Error on" lines, once per pass for NUM_RUNS = 100 passes, and
+    the same "Num Errors" count.  err_chk reads the reference's own prices, so
+    the test pins the check, not the pricing."""
+    import numpy as np
+    import pyoracle
+    from conftest import load_golden
+    exe = pyoracle.ref_binary("errchk")
+    if exe is None:
+        pytest.skip("the compiled reference (oracle/_ref) is not built here")
+    subprocess.run(["make", "-s", "-C", NATIVE, "parsec_io_check"], check=True)
+    book, ref = load_golden("config1_4096")
+    # 80 options, then 8 the reference prices at exactly 0 (deep out of the
+    # money), where DGrefval = +-float(1e-4) and +-the next float up give
+    # deltas exactly on and just over the line
+    zeros = np.flatnonzero(ref == 0.0)[:8]
+    assert zeros.size == 8
+    idx = np.concatenate([np.arange(80), zeros])
+    n = idx.size
+    price = ref[idx].astype(np.float32)
+    dg = price.copy()
+    f1e4 = np.float32(1e-4)
+    up = np.nextafter(f1e4, np.float32(1))
+    dg[80:88] = [f1e4, -f1e4, up, -up, f1e4, up, -f1e4, -up]
+    for j, i in enumerate(range(5, 80, 9)):              # clear errors, both signs
+        dg[i] = price[i] + np.float32(0.002 * (1 if j % 2 else -1))
+    inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
+    tchar = np.where(book.otype[idx] != 0, "P", "C")
+    with open(inp, "w") as f:
+        f.write(f"{n}\n")
+        for j, i in enumerate(idx):
+            f.write("%.9g %.9g %.9g 0 %.9g %.9g %s 0 %.9g\n" % (
+                book.s[i], book.k[i], book.r[i], book.v[i], book.t[i], tchar[j], dg[j]))
+    p = subprocess.run([exe, "1", str(inp), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    want = [ln for ln in p.stdout.splitlines() if ln.startswith(("Error on", "Num Errors"))]
+    q = subprocess.run([os.path.join(NATIVE, "parsec_io_check"), "--errchk", str(inp), str(out),
+                        "100"], capture_output=True, text=True, timeout=120)
+    assert q.returncode == 0, q.stdout + q.stderr
+    got = q.stdout.splitlines()
+    assert got == want
+    k = int(want[-1].split(":")[1])
+    assert k == 100 * (9 + 4) and len(want) == k + 1   # 9 clear errors, 4 just over the line

@@ -200,6 +200,9 @@ def test_static_blocks_thread_invariance():
     a = S.price_batch(sws, base, 2000, 16, nthreads=1)
     b = S.price_batch(sws, base, 2000, 16, nthreads=4)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # the pruned per-chain form (the CPU baseline's second variant): same bits
+    c = S.price_batch(sws, base, 2000, 16, nthreads=4, chain=True)
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
     for i in (0, 8):
         assert S.price_blocking(sws[i], base + i, 2000) == (a[0][i], a[1][i])
 

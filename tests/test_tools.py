@@ -53,6 +53,27 @@ def test_cli_reference_workflow(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_check_counts_errors_per_pass(gpu, tmp_path):
+    """--check with 7 options whose DGrefval is 2e-3 off the reference's price:
+    the reference's ERR_CHK counts and reports each once per pass, so 700
+    "Error on" lines and "Num Errors: 700" (the other prices are within
+    8.4e-5 of the reference, so none of them is flagged)."""
+    book, ref = load_golden("config1_4096")
+    dg = ref.astype(np.float32).copy()
+    bad = [3, 100, 777, 1500, 2048, 3333, 4095]
+    for j, i in enumerate(bad):
+        dg[i] += np.float32(2e-3 if j % 2 else -2e-3)
+    inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
+    write_input_with_ref(inp, book, dg)
+    p = subprocess.run([CLI, "8", str(inp), str(out), "--check"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("Error on")]
+    assert "Num Errors: 700" in p.stdout and len(lines) == 700
+    assert [int(ln.split()[2].rstrip(".")) for ln in lines[:7]] == bad
+
+
+@pytest.mark.gpu
 def test_compat_header_forms(gpu, oracle, tmp_path):
     book = books.make_book(1001, books.SEED_BASE + 60)
     ref, _, _ = oracle.price_f32(book)
