@@ -200,7 +200,9 @@ typedef struct {
  * sums divided by trials).  Every path of every swaption runs on the GPU.
  * Host arrays; synchronous on `stream`.  Limits: trials in [1, 2^40],
  * block_size >= 1, seed in [0, 2^42) and seed + n + paths * (n_steps - 1) *
- * n_factors below 2^42, the swap's start step x n_factors <= 64; a record
+ * n_factors below 2^42, at most 2^24 blocks of 1024 paths in one call
+ * (n * ceil(paths / 1024) <= 2^24, about 2^34 paths in all: one launch of at
+ * most 2^32 work-items), the swap's start step x n_factors <= 64; a record
  * the reference would index out of its arrays with (maturity past the
  * horizon, tenor past the curve, payment interval under one step) is
  * BS_ERR_INVALID.  A call's device buffers (about 1.5 KB per swaption plus

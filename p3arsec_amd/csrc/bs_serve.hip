@@ -167,6 +167,7 @@ struct HipExec {
     unsigned section_grid[kSlots][2] = {};      // grid the exec's section node has now
     void* section_args[kSlots][1] = {};         // kernel arguments: &dev_slot[s]
     hipEvent_t ev[kSlots] = {};
+    hipStream_t slot_stream[kSlots] = {};        // the stream slot s was last launched on
     int64_t seq = 0, nlaunch = 0;
 
     hipError_t init(int dev, size_t arena_bytes, int graph_on, int grid_blocks) {
@@ -298,6 +299,7 @@ struct HipExec {
     int launch(int s, const std::vector<serve::Admitted>& reqs, std::string* err) {
         write_slot(s, reqs);
         hipStream_t st = s < kSmallSlots ? streams[2] : streams[nlaunch++ & 1];
+        slot_stream[s] = st;
         const SlotHeader* h = reinterpret_cast<const SlotHeader*>(host_slot[s]);
         hipError_t e = hipSuccess;
         if (graph[s]) {
@@ -331,6 +333,12 @@ struct HipExec {
         if (q == hipSuccess) return 1;
         *err = hipGetErrorString(q);
         return -1;
+    }
+    // A launch that failed part-way may have queued the stage kernel or a
+    // section on the slot's stream: wait for that stream before the loop frees
+    // the batch's outputs and reuses the slot's tables.
+    void quiesce(int s) {
+        if (slot_stream[s]) (void)hipStreamSynchronize(slot_stream[s]);
     }
     void drain() {
         if (stream) (void)hipStreamSynchronize(stream);

@@ -505,7 +505,10 @@ extern "C" int bs_swaptions_host_f64(const bs_swaption_t* sw, int64_t n, int64_t
         if (pool.size() > (size_t)INT32_MAX) return set_error(BS_ERR_INVALID, "too many swaptions");
         const int64_t per_block = (int64_t)kSwThreads * kSwPer;
         const int64_t bps64 = (paths + per_block - 1) / per_block;
-        if (bps64 * n > 0x7fffffff) return set_error(BS_ERR_INVALID, "too many paths");
+        // one launch of at most 2^32 work-items: 2^24 blocks of 1024 paths
+        if (bps64 * n > (int64_t(1) << 24))
+            return set_error(BS_ERR_INVALID,
+                             "more than 2^24 blocks of 1024 paths (2^34 paths) in one call");
         const int bps = (int)bps64;
         const unsigned blocks = (unsigned)(bps64 * n);
         const hipStream_t st = static_cast<hipStream_t>(stream);

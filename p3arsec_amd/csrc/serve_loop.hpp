@@ -16,6 +16,10 @@
 //   void thread_start();                               on the dispatcher thread
 //   int launch(int slot, const std::vector<Admitted>&, std::string* err);   0 = ok
 //   int query(int slot, std::string* err);             0 running, 1 done, -1 failed
+//   void quiesce(int slot);                            after a failed launch: waits
+//                                                      until nothing the launch
+//                                                      enqueued before failing can
+//                                                      still run
 //   void drain();                                      waits for every launched batch
 //
 // Lanes: with Limits::small_options > 0, requests of at most that many
@@ -367,6 +371,9 @@ class Loop {
                 const int e = ex_->launch(s, reqs, &err);
                 t_launch_.fetch_add(steady_ns() - b.t_launch, std::memory_order_relaxed);
                 if (e != 0) {
+                    // part of the batch may be queued on the device already: it
+                    // must be done before its outputs and slot are handed out again
+                    ex_->quiesce(s);
                     finish_batch(b.tickets, false, "launch failed: " + err, -1);
                 } else {
                     busy[s] = true;

@@ -43,7 +43,7 @@ struct HostExec {
     int fail_launch_every = 0, fail_batch_every = 0;
     int64_t fixed_ns = 20000;       // per batch
     double ns_per_option = 0.002;   // ~ 5e11 options/s
-    int64_t launches = 0, batches = 0;
+    int64_t launches = 0, batches = 0, partial_launches = 0;
     int64_t small_options = 0;   // the loop's lane threshold, checked per launch
     struct Slot {
         bool used = false, fail = false;
@@ -59,10 +59,10 @@ struct HostExec {
         CHECK(s >= 0 && s < kSlots && !slot[s].used);
         CHECK(!reqs.empty());
         ++launches;
-        if (fail_launch_every && launches % fail_launch_every == 0) {
-            *err = "injected launch failure";
-            return -1;
-        }
+        // an injected launch failure happens after the batch's work was queued
+        // (as when a later kernel launch or the event record fails): its
+        // outputs stay in flight until quiesce()
+        const bool fail_launch = fail_launch_every && launches % fail_launch_every == 0;
         Slot& sl = slot[s];
         sl.outs.clear();
         int64_t options = 0;
@@ -100,7 +100,19 @@ struct HostExec {
         sl.used = true;
         sl.fail = fail_batch_every && batches % fail_batch_every == 0;
         sl.done_at = steady_ns() + fixed_ns + (int64_t)(ns_per_option * (double)options);
+        if (fail_launch) {
+            ++partial_launches;
+            *err = "injected launch failure";
+            return -1;
+        }
         return 0;
+    }
+    void quiesce(int s) {
+        Slot& sl = slot[s];
+        if (!sl.used) return;
+        while (steady_ns() < sl.done_at) std::this_thread::yield();
+        for (const auto& o : sl.outs) inflight.erase(o.first);
+        sl.used = false;
     }
     int query(int s, std::string* err) {
         Slot& sl = slot[s];
@@ -241,9 +253,11 @@ int scenario_faults() {
     w.loop->stats(&st, &cap, &used, &peak, &lt);
     CHECK(failed > 0 && st.requests_failed == failed);
     CHECK(st.requests_done + st.requests_failed + st.requests_cancelled == fin);
+    CHECK(w.ex.partial_launches > 0);   // launches that failed with work already queued
     check_drained(w);   // failed batches returned their outputs
-    printf("faults: %lld requests, %lld failed, %lld launches\n", (long long)fin, (long long)failed,
-           (long long)st.launches);
+    printf("faults: %lld requests, %lld failed, %lld launches (%lld failed part-way)\n",
+           (long long)fin, (long long)failed, (long long)st.launches,
+           (long long)w.ex.partial_launches);
     return 0;
 }
 

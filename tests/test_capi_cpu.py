@@ -111,6 +111,12 @@ def test_swaptions_argument_errors_before_device_work():
     assert rc(sws, seed=2 ** 42) == -1
     assert rc(sws, trials=2 ** 38) == -1               # counters past 2^42
     assert b"2^42" in L.bs_last_error()
+    # one launch holds at most 2^24 blocks of 1024 paths: 2^34 + 1 trials of one
+    # swaption need one block more (refused before any device work)
+    assert rc(sws[:1], trials=2 ** 34 + 1) == -1
+    assert b"2^24 blocks" in L.bs_last_error()
+    assert rc(sws, trials=2 ** 33 + 1) == -1            # two swaptions: 2 x (2^23 + 1) blocks
+    assert b"2^24 blocks" in L.bs_last_error()
     assert rc(mod(maturity=30.0)) == -1                 # start past the horizon
     assert b"horizon" in L.bs_last_error()
     assert rc(mod(interval=0.01)) == -1                 # payments under one step
