@@ -29,6 +29,7 @@ STATUS = {
 }
 CFG_CHECK_POINTERS, CFG_MAX_BLOCKS, CFG_BLOCK_THREADS, CFG_VECTOR_WIDTH, CFG_NONTEMPORAL = 1, 2, 3, 4, 5
 CFG_PIPELINE = 6
+CFG_LDS_PER_WAVE = 7
 REQ_PRICE, REQ_GREEKS = 1, 2
 TICKET_QUEUED, TICKET_RUNNING, TICKET_DONE, TICKET_FAILED, TICKET_CANCELLED = range(5)
 

@@ -139,7 +139,8 @@ struct PassKey {
     const void* ptr[7];
     int64_t n;
     int passes, type_bytes, device;
-    int64_t launch[5];   // max_blocks, vector_width, nontemporal, pipeline, block_threads
+    int64_t launch[6];   // max_blocks, vector_width, nontemporal, pipeline, block_threads,
+                         // lds_per_wave
     bool operator==(const PassKey& o) const {
         return memcmp(ptr, o.ptr, sizeof(ptr)) == 0 && n == o.n && passes == o.passes &&
                type_bytes == o.type_bytes && device == o.device &&
@@ -259,6 +260,11 @@ int bs_config_set(int key, int64_t value) {
                 return set_error(BS_ERR_INVALID, "block threads must be 0, 64, 128 or 256");
             config().block_threads = (int)value;
             return BS_OK;
+        case BS_CFG_LDS_PER_WAVE:
+            if (value < -1 || value > 65536)
+                return set_error(BS_ERR_INVALID, "LDS per wave must be -1, 0 or 1..65536 bytes");
+            config().lds_per_wave = (int)value;
+            return BS_OK;
         default: return set_error(BS_ERR_INVALID, "unknown config key");
     }
 }
@@ -272,6 +278,7 @@ int bs_config_get(int key, int64_t* value) {
         case BS_CFG_VECTOR_WIDTH: *value = config().vector_width; return BS_OK;
         case BS_CFG_NONTEMPORAL: *value = config().nontemporal; return BS_OK;
         case BS_CFG_PIPELINE: *value = config().pipeline; return BS_OK;
+        case BS_CFG_LDS_PER_WAVE: *value = config().lds_per_wave; return BS_OK;
         default: return set_error(BS_ERR_INVALID, "unknown config key");
     }
 }
@@ -398,6 +405,7 @@ int bs_portfolio_f32(const float* s, const float* k, const float* r, const float
                 "bs_portfolio_f32");
 }
 
+
 int bs_repack_aos_f32(const void* aos, int64_t n, float* s, float* k, float* r, float* v,
                       float* t, uint8_t* otype, bs_stream_t stream) {
     int rc;
@@ -436,7 +444,7 @@ int bs_price_passes_f32(const float* s, const float* k, const float* r, const fl
             const Config& cf = config();
             const PassKey key{{s, k, r, v, t, otype, price}, n, passes, otype_bytes, dev,
                               {cf.max_blocks, cf.vector_width, cf.nontemporal, cf.pipeline,
-                               cf.block_threads}};
+                               cf.block_threads, cf.lds_per_wave}};
             PassGraphCache& c = pass_cache();
             std::lock_guard<std::mutex> lk(c.mu);
             PassGraph* g = pass_graph(c, key, in, otype, price, &e);

@@ -23,6 +23,7 @@ struct Config {
     std::atomic<int> nontemporal{1};
     std::atomic<int> pipeline{-1};       // -1: each kernel's measured default
     std::atomic<int> block_threads{0};   // streaming kernels' block size: 0 = each kernel's default
+    std::atomic<int> lds_per_wave{-1};   // LDS reserved per wave: -1 = each kernel's default
 };
 Config& config();
 

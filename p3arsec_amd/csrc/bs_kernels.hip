@@ -126,11 +126,15 @@ hipError_t launch_form(const Op& op, int64_t n, hipStream_t st) {
     const int64_t cap = mb > 0 ? mb : Op::kBlocks;
     const int nt = bt > 0 ? bt : Op::kThreadsPref;
     const unsigned grid = grid_for(std::max<int64_t>(ngroups, 1), cap, nt);
+    // LDS reserved per block caps the waves resident on a CU (none is used)
+    const int lw = config().lds_per_wave.load(std::memory_order_relaxed);
+    const size_t lds = std::min<size_t>(
+        (size_t)(lw < 0 ? Op::kLdsPerWave : lw) * (size_t)((nt + 63) / 64), 65536);
     if (n < (int64_t(1) << 32) / Op::kElemBytes)
-        hipLaunchKernelGGL((k_stream<Op, NT, PIPE, uint32_t>), dim3(grid), dim3(nt), 0, st, op,
+        hipLaunchKernelGGL((k_stream<Op, NT, PIPE, uint32_t>), dim3(grid), dim3(nt), lds, st, op,
                            ngroups, ngroups * VW, n);
     else
-        hipLaunchKernelGGL((k_stream<Op, NT, PIPE, int64_t>), dim3(grid), dim3(nt), 0, st, op,
+        hipLaunchKernelGGL((k_stream<Op, NT, PIPE, int64_t>), dim3(grid), dim3(nt), lds, st, op,
                            ngroups, ngroups * VW, n);
     return hipGetLastError();
 }

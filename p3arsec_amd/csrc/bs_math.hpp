@@ -153,26 +153,32 @@ __device__ __forceinline__ float cndf_grad(float ax, float k, float poly, float 
 template <bool kChecked>
 struct Ref32Arith {
     using T = float;
+    static constexpr bool kSplitCndf = true;
     __device__ static __forceinline__ float exp(float x) { return exp_hw(x); }
     __device__ static __forceinline__ float log(float x) { return log_hw(x); }
     __device__ static __forceinline__ float div(float a, float b) { return div_rn<kChecked>(a, b); }
     __device__ static __forceinline__ float sqrt(float x) { return sqrt_rn(x); }
 
-    // blackscholes.c:126-184, statement by statement.
+    // blackscholes.c:126-184, statement by statement: y = 1 - tail(|x|), the
+    // value for x >= 0; CNDF(x) is y for x >= 0 and 1 - y for x < 0
+    // (:143-147, :179-181).
     template <bool kGrad>
-    __device__ static __forceinline__ float cndf(float x, float* grad) {
+    __device__ static __forceinline__ float cndf_abs(float x, float* grad) {
 #pragma clang fp contract(off)
         constexpr SplitConst c0 = split(kInvSqrt2Pi), cp = split(kP), b1 = split(kB1),
                              b2 = split(kB2), b3 = split(kB3), b4 = split(kB4),
                              b5 = split(kB5);
-        const bool neg = x < 0.0f;
         // |x| capped at 1e10: for every |x| >= 20 the reference returns exactly
         // 1 (e^{-x^2/2} is 0 or denormal and the tail rounds away), and so does
         // the capped value; uncapped, |x| = inf would meet inf*hi + inf*lo = NaN
         // in mul_dc where the reference's double product is inf.  NaN passes.
         float ax = __builtin_fabsf(x);   // |-0| = +0 gives the same bits below
         if constexpr (kChecked) ax = ax > 1e10f ? 1e10f : ax;
-        float n = exp(-0.5f * ax * ax);
+        // e^{-x^2/2} (:152): the reference's exponent (-0.5 x) x, times log2 e
+        // for v_exp_f32, rounds twice; -0.5 scales exactly, so x^2 times the
+        // one constant -0.5 log2 e rounds to the same float for every x (all
+        // 2^32 inputs checked on the MI355X)
+        float n = __builtin_amdgcn_exp2f((ax * ax) * (-0.5f * kLog2e));
         n = mul_dc(n, c0);
         float k = mul_dc(ax, cp);
         k = 1.0f + k;
@@ -190,7 +196,13 @@ struct Ref32Arith {
         const float tail = t1 * n;
         const float y = 1.0f - tail;
         if constexpr (kGrad) *grad = cndf_grad(ax, k, t1, n);
-        return neg ? 1.0f - y : y;
+        return y;
+    }
+
+    template <bool kGrad>
+    __device__ static __forceinline__ float cndf(float x, float* grad) {
+        const float y = cndf_abs<kGrad>(x, grad);
+        return x < 0.0f ? 1.0f - y : y;
     }
 };
 
@@ -203,6 +215,7 @@ struct Ref32 : Ref32Arith<true> {
 struct Fast32 {
     using T = float;
     using Unchecked = Fast32;
+    static constexpr bool kSplitCndf = false;
     __device__ static __forceinline__ float exp(float x) {
         return __builtin_amdgcn_exp2f(x * kLog2e);
     }
@@ -300,6 +313,7 @@ __device__ __forceinline__ double log64(double x) {
 struct Ref64 {
     using T = double;
     using Unchecked = Ref64;
+    static constexpr bool kSplitCndf = false;
     __device__ static __forceinline__ double exp(double x) { return ::exp(x); }
     __device__ static __forceinline__ double log(double x) { return log64(x); }
     __device__ static __forceinline__ double div(double a, double b) { return a / b; }
@@ -331,6 +345,7 @@ struct Ref64 {
 template <class T>
 struct Forward {
     T sqrt_t, half_var, den, d1, d2, n1, n2, g1, g2, disc, fv;
+    T y1, y2;   // Ref32Arith: CNDF's non-negative-side values, n_i = d_i < 0 ? 1 - y_i : y_i
 };
 
 template <class M, bool kGrad>
@@ -351,8 +366,15 @@ __device__ __forceinline__ Forward<typename M::T> bs_forward(typename M::T s, ty
     f.den = v * f.sqrt_t;                      // :238
     f.d1 = M::div(d1, f.den);                  // :239
     f.d2 = f.d1 - f.den;                       // :240
-    f.n1 = M::template cndf<kGrad>(f.d1, &f.g1);   // :245-246
-    f.n2 = M::template cndf<kGrad>(f.d2, &f.g2);
+    if constexpr (M::kSplitCndf) {                  // :245-246
+        f.y1 = M::template cndf_abs<kGrad>(f.d1, &f.g1);
+        f.y2 = M::template cndf_abs<kGrad>(f.d2, &f.g2);
+        f.n1 = f.d1 < T(0) ? T(1) - f.y1 : f.y1;
+        f.n2 = f.d2 < T(0) ? T(1) - f.y2 : f.y2;
+    } else {
+        f.n1 = M::template cndf<kGrad>(f.d1, &f.g1);
+        f.n2 = M::template cndf<kGrad>(f.d2, &f.g2);
+    }
     f.disc = M::exp(-r * t);                   // :248
     f.fv = k * f.disc;
     return f;
@@ -368,6 +390,25 @@ __device__ __forceinline__ T bs_combine(T s, const Forward<T>& f, bool put) {
     return f.fv * m2 - s * m1;
 }
 
+// The same price from the CNDF halves (M::kSplitCndf), in fewer operations.
+// The put's 1 - N(d) is 1 - y for d >= 0 and 1 - (1 - y) for d < 0, and the
+// latter is y exactly: 1 - y is exact for y in [0.5, 1] (Sterbenz), and for
+// the few y just below 0.5 that CNDF returns near d = 0, 1 - y is exact too
+// (every float y it returns was checked on the MI355X), so the rounded
+// difference 1 - (1 - y) is the float y itself.  So both option types
+// multiply S and FV by a_i = (d_i < 0) != put ? 1 - y_i : y_i, and
+// price = put ? FV a2 - S a1 : S a1 - FV a2, the reference's products and
+// difference (:250, :252-254) without a branch per option type.
+template <class T>
+__device__ __forceinline__ T bs_combine_split(T s, const Forward<T>& f, bool put) {
+#pragma clang fp contract(off)
+    const T a1 = ((f.d1 < T(0)) != put) ? T(1) - f.y1 : f.y1;
+    const T a2 = ((f.d2 < T(0)) != put) ? T(1) - f.y2 : f.y2;
+    const T x = s * a1;
+    const T z = f.fv * a2;
+    return put ? z - x : x - z;
+}
+
 // A policy with a distinct unchecked form prices with it first and recomputes
 // with the checked one where the price is NaN (Ref32: identical bits).
 template <class M>
@@ -381,7 +422,7 @@ __device__ __forceinline__ typename M::T bs_price(typename M::T s, typename M::T
                                                   typename M::T t, bool put) {
     if constexpr (has_unchecked<M>()) {
         using U = typename M::Unchecked;
-        typename M::T p = bs_combine(s, bs_forward<U, false>(s, k, r, v, t), put);
+        typename M::T p = bs_combine_split(s, bs_forward<U, false>(s, k, r, v, t), put);
         if (__builtin_expect(p != p, 0)) p = bs_combine(s, bs_forward<M, false>(s, k, r, v, t), put);
         return p;
     } else {
@@ -400,13 +441,14 @@ struct Greeks32 {
 //   den = sigma sqrt(T),  A = (r + sigma^2/2) T + log(S/K).
 // Outputs: delta = dP/dS, vega = dP/dsigma, rho = dP/dr, theta = -dP/dT (per
 // year); vega and rho are per unit change, not per percent.
-template <class M>
+template <class M, bool kSplitCombine>
 __device__ __forceinline__ Greeks32 bs_greeks_of(float s, float k, float r, float v, float t,
                                                  bool put) {
 #pragma clang fp contract(off)
     const Forward<float> f = bs_forward<M, true>(s, k, r, v, t);
     Greeks32 o;
-    o.price = bs_combine(s, f, put);
+    if constexpr (kSplitCombine) o.price = bs_combine_split(s, f, put);
+    else o.price = bs_combine(s, f, put);
     const float rden = hw_rcp(f.den);
     // adjoints of N1 and N2 are S and -FV for both option types
     const float s_bar0 = put ? f.n1 - 1.0f : f.n1;
@@ -432,11 +474,12 @@ template <class M>
 __device__ __forceinline__ Greeks32 bs_greeks(float s, float k, float r, float v, float t,
                                               bool put) {
     if constexpr (has_unchecked<M>()) {
-        Greeks32 o = bs_greeks_of<typename M::Unchecked>(s, k, r, v, t, put);
-        if (__builtin_expect(o.price != o.price, 0)) o = bs_greeks_of<M>(s, k, r, v, t, put);
+        Greeks32 o = bs_greeks_of<typename M::Unchecked, true>(s, k, r, v, t, put);
+        if (__builtin_expect(o.price != o.price, 0))
+            o = bs_greeks_of<M, false>(s, k, r, v, t, put);
         return o;
     } else {
-        return bs_greeks_of<M>(s, k, r, v, t, put);
+        return bs_greeks_of<M, false>(s, k, r, v, t, put);
     }
 }
 

@@ -5,16 +5,22 @@
 // Each op: Group (registers of one VW-option group), load<NT>(g),
 // finish<NT>(g, x) (compute + store; g is the group index, int64_t or, for
 // arrays under 4 GiB, uint32_t), scalar(i) for options outside whole groups,
-// kElemBytes (the widest array element, for the 4 GiB test), kWaves (the occupancy the register allocator is held to: 8 waves
+// kElemBytes (the widest array element, for the 4 GiB test), kLdsPerWave
+// (LDS bytes reserved per wave of a block, which caps the waves resident on a
+// CU: 160 KB / kLdsPerWave; write-heavy streams run faster with fewer waves
+// in flight), kWaves (the occupancy the register allocator is held to: 8 waves
 // per SIMD = 64 VGPRs), kBlocks (default grid cap, 0 = one group per
 // work-item) and kPipe (default loop form); the defaults and each launcher's
 // preferred width are the fastest forms measured on MI355X for 10M options.
-// Defaults measured on MI355X, 10M options, warm clocks (block size x loop
-// form): price 64 threads pipelined 39.3 us (256 threads: 43.0), int32-type
-// price 64 threads 44.9 us, FP64 64 threads pipelined 108 us, Greeks 256
-// threads pipelined 70.7 us (64 threads: 74.2).  Small blocks help the
-// price streams: one-wave workgroups retire and refill independently, so the
-// tail of the grid is short.
+// Defaults measured on MI355X, 10M options, warm clocks, passes on one / two
+// streams: price 64 threads, 6 KB of LDS per wave 39.2 / 38.1 us (no LDS
+// reserved: 40.8 / 38.2; 256 threads: 43.4 / 42.2); Greeks 64 threads, 11 KB
+// per wave 68.0 / 67.0 us (no LDS: 74.1 / 74.5; 256 threads, no LDS: 71.2 /
+// 74.7); int32-type price 64 threads 44.9 us, FP64 64 threads pipelined
+// 108 us.  Small blocks help: one-wave workgroups retire and refill
+// independently, so the tail of the grid is short.  Fewer resident waves help
+// the write-heavy Greeks stream most: with 14 waves per CU instead of 32 its
+// five output streams keep fewer DRAM pages open at once.
 #pragma once
 #include <type_traits>
 
@@ -64,6 +70,7 @@ template <class Math, int VW>
 struct PriceF32 {
     static constexpr int kVW = VW;
     static constexpr int kElemBytes = 4;   // widest array element
+    static constexpr int kLdsPerWave = 6144;   // 26 waves per CU
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
@@ -104,6 +111,7 @@ template <int VW>
 struct PriceI32F32 {
     static constexpr int kVW = VW;
     static constexpr int kElemBytes = 4;   // widest array element
+    static constexpr int kLdsPerWave = 0;
     static constexpr int kWaves = 7;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
@@ -146,6 +154,7 @@ template <int VW>
 struct PriceF64 {
     static constexpr int kVW = 2;
     static constexpr int kElemBytes = 8;   // widest array element
+    static constexpr int kLdsPerWave = 0;
     static constexpr int kWaves = 5;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
@@ -186,8 +195,9 @@ template <int VW>
 struct GreeksF32 {
     static constexpr int kVW = VW;
     static constexpr int kElemBytes = 4;   // widest array element
+    static constexpr int kLdsPerWave = 11264;  // 14 waves per CU
     static constexpr int kWaves = VW == 4 ? 5 : 8;
-    static constexpr int kThreadsPref = 256;
+    static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
     static constexpr bool kPipe = true;
     using F = vec<float, VW>;
@@ -243,6 +253,7 @@ template <int VW>
 struct CndfF32 {
     static constexpr int kVW = VW;
     static constexpr int kElemBytes = 4;   // widest array element
+    static constexpr int kLdsPerWave = 0;
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;
@@ -273,6 +284,7 @@ template <int VW>
 struct CndfF64 {
     static constexpr int kVW = 2;
     static constexpr int kElemBytes = 8;   // widest array element
+    static constexpr int kLdsPerWave = 0;
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 256;
     static constexpr int64_t kBlocks = 0;

@@ -47,6 +47,12 @@ def test_config_roundtrip_and_validation():
     assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 100) == -1  # 0, 64, 128 or 256
     assert L.bs_config_set(capi.CFG_BLOCK_THREADS, 512) == -1
     capi.set_config(capi.CFG_BLOCK_THREADS, 0)
+    assert capi.get_config(capi.CFG_LDS_PER_WAVE) == -1        # each kernel's default
+    capi.set_config(capi.CFG_LDS_PER_WAVE, 4096)
+    assert capi.get_config(capi.CFG_LDS_PER_WAVE) == 4096
+    assert L.bs_config_set(capi.CFG_LDS_PER_WAVE, 65537) == -1
+    assert L.bs_config_set(capi.CFG_LDS_PER_WAVE, -2) == -1
+    capi.set_config(capi.CFG_LDS_PER_WAVE, -1)
     assert L.bs_config_set(999, 1) == -1
 
 
