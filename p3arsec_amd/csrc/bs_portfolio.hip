@@ -3,22 +3,27 @@
 //
 // sums[0] = sum_i w_i * price_i and, with Greeks, sums[1..4] = the same sums of
 // delta, vega, rho and theta, where each per-option value is exactly the one
-// the map computes (the same per-group code, bs_ops.hpp) and w_i is the
+// the map computes (the same per-option code, bs_math.hpp) and w_i is the
 // position (1 when no weights are given).
 //
-// Three kernels on the caller's stream.  k_pf_waves has the map's shape: one
-// 2-option group per work-item (64-thread blocks for prices, 256 for Greeks,
-// as the maps); each work-item forms w*value in FP64 and each wave folds its
-// lanes with a fixed butterfly into one partial.  k_pf_chunks: each block adds
-// a contiguous chunk of 2048 partials (eight per work-item, in order, then a
-// fixed tree).  k_pf_final: one block adds the chunk sums the same way.  All
-// grids depend only on n, so the sums are bit-reproducible for a given book.
-// The partials live in stream-ordered memory (hipMallocAsync/hipFreeAsync), so
-// calls on different streams never share them.
+// Two kernels on the caller's stream.  k_pf_rows: wave w of the grid owns
+// kPfRows consecutive rows of 64 two-option groups (the map's loads: one
+// 8-byte load per array per lane, so each wave instruction reads 512
+// contiguous bytes); a lane walks its column of those rows in order, the next
+// row's loads issued before the current row's arithmetic, and adds w*value to
+// FP64 accumulators; the wave then folds its lanes with a fixed butterfly into
+// one partial.  k_pf_final: one block of 1024 work-items adds the partials
+// (work-item j takes partials j, j + 1024, ... in order, then a fixed tree);
+// past 65536 partials (books over 2^29 options), k_pf_chunks blocks first add
+// contiguous chunks of 2048.  Every grid depends only on n, so the sums are
+// bit-reproducible for a given book.
+//
+// Scratch (the partials and the chunk sums) comes from stream-ordered memory
+// (hipMallocAsync/hipFreeAsync), so calls on different streams never share it.
 //
 // Traffic: 21 bytes per option read (25 with weights), 8 or 40 bytes written
-// in all; the price form is bound by HBM reads and VALU together, the Greeks
-// form by VALU.
+// in all, plus 8 or 40 bytes of partials per kPfRows x 128 options.
+#include <algorithm>
 #include <type_traits>
 
 #include "bs_internal.hpp"
@@ -28,6 +33,10 @@
 namespace bs {
 namespace {
 
+constexpr int kPfRows = 8;                                      // rows of 64 groups per wave
+// LDS reserved per wave (caps the resident waves): price form none, Greeks
+// form 8 KB (20 waves per CU; 10M options 46.0 us against 47.9 with none)
+constexpr int pf_lds_per_wave(bool greeks) { return greeks ? 8192 : 0; }
 constexpr int kPfChunkThreads = 256;
 constexpr int kPfChunkPer = 8;                                  // partials per work-item
 constexpr int64_t kPfChunk = kPfChunkThreads * kPfChunkPer;     // partials per chunk block
@@ -37,7 +46,7 @@ struct PfArgs {
     const uint8_t* type;
     const float* w;   // NULL: every weight is 1
     int64_t n;
-    double* partial;  // [gridDim.x][NQ]
+    double* partial;  // [waves][NQ]
 };
 
 template <bool kGreeks>
@@ -56,49 +65,77 @@ __device__ __forceinline__ void add_option(double* acc, float s, float k, float 
     }
 }
 
-// VW = 2: aligned arrays, one 2-option group per work-item; VW = 1: any
-// alignment, one option per work-item.
-template <int VW, bool kGreeks, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kGreeks ? 6 : 8)))
-void k_pf_waves(PfArgs a) {
+// One group's inputs: VW = 2 options of aligned arrays (8-byte loads), or
+// VW = 1 option of any alignment.  Ix = uint32_t when every array of the call
+// spans less than 4 GiB (32-bit offsets on the arrays' bases), else int64_t.
+template <int VW>
+struct PfGroup {
+    using F = vec<float, VW>;
+    F s, k, r, v, t, w;
+    uint32_t ty;
+};
+
+template <int VW, class Ix>
+__device__ __forceinline__ PfGroup<VW> pf_load(const PfArgs& a, Ix g) {
+    using F = vec<float, VW>;
+    PfGroup<VW> x;
+    x.s = __builtin_nontemporal_load(at(reinterpret_cast<const F*>(a.in.s), g));
+    x.k = __builtin_nontemporal_load(at(reinterpret_cast<const F*>(a.in.k), g));
+    x.r = __builtin_nontemporal_load(at(reinterpret_cast<const F*>(a.in.r), g));
+    x.v = __builtin_nontemporal_load(at(reinterpret_cast<const F*>(a.in.v), g));
+    x.t = __builtin_nontemporal_load(at(reinterpret_cast<const F*>(a.in.t), g));
+    if constexpr (VW == 2)
+        x.ty = __builtin_nontemporal_load(at(reinterpret_cast<const uint16_t*>(a.type), g));
+    else
+        x.ty = __builtin_nontemporal_load(at(a.type, g));
+    if (a.w) x.w = __builtin_nontemporal_load(at(reinterpret_cast<const F*>(a.w), g));
+    else x.w = F(1.0f);
+    return x;
+}
+
+template <int VW, bool kGreeks>
+__device__ __forceinline__ void pf_add(double* acc, const PfGroup<VW>& x) {
+#pragma unroll
+    for (int j = 0; j < VW; ++j)
+        add_option<kGreeks>(acc, x.s[j], x.k[j], x.r[j], x.v[j], x.t[j], put_of(x.ty, j), x.w[j]);
+}
+
+// Groups of the call: n / VW whole groups, plus (VW = 2, n odd) the last
+// option alone, priced by the lane that would own group n / 2.
+template <int VW, bool kGreeks, int NT, class Ix>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kGreeks ? 4 : 5)))
+void k_pf_rows(PfArgs a) {
     constexpr int NQ = kGreeks ? 5 : 1;
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    const int64_t gid = (int64_t)blockIdx.x * NT + threadIdx.x;
-    if constexpr (VW == 2) {
-        using F = vec<float, 2>;
-        if (gid < a.n / 2) {
-            const int64_t g = gid;
-            const F s = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.s) + g);
-            const F k = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.k) + g);
-            const F r = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.r) + g);
-            const F v = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.v) + g);
-            const F t = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.in.t) + g);
-            const uint16_t ty = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(a.type) + g);
-            F w = {1.0f, 1.0f};
-            if (a.w) w = __builtin_nontemporal_load(reinterpret_cast<const F*>(a.w) + g);
-            add_option<kGreeks>(acc, s[0], k[0], r[0], v[0], t[0], put_of(ty, 0), w[0]);
-            add_option<kGreeks>(acc, s[1], k[1], r[1], v[1], t[1], put_of(ty, 1), w[1]);
-        } else if (gid == a.n / 2 && (a.n & 1)) {   // the last option of an odd book
-            const int64_t i = a.n - 1;
-            add_option<kGreeks>(acc, a.in.s[i], a.in.k[i], a.in.r[i], a.in.v[i], a.in.t[i],
-                                a.type[i] != 0, a.w ? a.w[i] : 1.0f);
+    const Ix ngroups = (Ix)(VW == 2 ? a.n / 2 : a.n);
+    const Ix lane = threadIdx.x & 63;
+    const Ix wave = (Ix)blockIdx.x * (NT / 64) + threadIdx.x / 64;
+    Ix g = wave * (64 * kPfRows) + lane;
+    const Ix g_end = g + 64 * kPfRows;
+    if (g < ngroups) {
+        PfGroup<VW> cur = pf_load<VW>(a, g);
+        for (Ix gn = g + 64; gn < g_end && gn < ngroups; gn += 64) {
+            const PfGroup<VW> nxt = pf_load<VW>(a, gn);
+            pf_add<VW, kGreeks>(acc, cur);
+            cur = nxt;
+            g = gn;
         }
-    } else {
-        if (gid < a.n) {
-            const int64_t i = gid;
-            add_option<kGreeks>(acc, a.in.s[i], a.in.k[i], a.in.r[i], a.in.v[i], a.in.t[i],
-                                a.type[i] != 0, a.w ? a.w[i] : 1.0f);
-        }
+        pf_add<VW, kGreeks>(acc, cur);
+        g += 64;
     }
-    const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64;
+    if (VW == 2 && (a.n & 1) && g == ngroups && g < g_end) {   // the last option of an odd book
+        const int64_t i = a.n - 1;
+        add_option<kGreeks>(acc, a.in.s[i], a.in.k[i], a.in.r[i], a.in.v[i], a.in.t[i],
+                            a.type[i] != 0, a.w ? a.w[i] : 1.0f);
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double x = acc[q];
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
-        if ((threadIdx.x & 63) == 0) a.partial[wave * NQ + q] = x;
+        if (lane == 0) a.partial[(int64_t)wave * NQ + q] = x;
     }
 }
 
@@ -135,11 +172,41 @@ __global__ __launch_bounds__(kPfChunkThreads) void k_pf_chunks(const double* __r
     if (threadIdx.x < NQ) out[(int64_t)blockIdx.x * NQ + threadIdx.x] = lds[threadIdx.x][0];
 }
 
+// One block sums count <= kPfFinalMax partials: work-item j adds partials j,
+// j + 1024, j + 2048, ... in order, then a fixed tree over the block.
+constexpr int kPfFinalThreads = 1024;
+constexpr int64_t kPfFinalMax = int64_t(kPfFinalThreads) * 64;
+
+template <int NQ>
+__global__ __launch_bounds__(kPfFinalThreads) void k_pf_final(const double* __restrict__ in,
+                                                              int64_t count,
+                                                              double* __restrict__ out) {
+    __shared__ double lds[NQ][kPfFinalThreads];
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int64_t i = threadIdx.x; i < count; i += kPfFinalThreads)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] += in[i * NQ + q];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) lds[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int half = kPfFinalThreads / 2; half >= 1; half >>= 1) {
+        if (threadIdx.x < half)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) lds[q][threadIdx.x] += lds[q][threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x < NQ) out[threadIdx.x] = lds[threadIdx.x][0];
+}
+
 // Reduces count partials (count >= 1) to out[0..NQ) in a fixed order: chunk
-// sums until one is left.  tmp holds at least ceil(count / kPfChunk) entries.
+// sums while more than kPfFinalMax are left, then one final block.  tmp holds
+// ceil(count / kPfChunk) entries per chunk level (pf_scratch_doubles counts
+// them).
 template <int NQ>
 void reduce_partials(const double* in, int64_t count, double* tmp, double* out, hipStream_t st) {
-    while (count > kPfChunk) {
+    while (count > kPfFinalMax) {
         const int64_t nb = (count + kPfChunk - 1) / kPfChunk;
         hipLaunchKernelGGL((k_pf_chunks<NQ>), dim3((unsigned)nb), dim3(kPfChunkThreads), 0, st,
                            in, count, tmp);
@@ -147,43 +214,77 @@ void reduce_partials(const double* in, int64_t count, double* tmp, double* out, 
         tmp += nb * NQ;
         count = nb;
     }
-    hipLaunchKernelGGL((k_pf_chunks<NQ>), dim3(1), dim3(kPfChunkThreads), 0, st, in, count, out);
+    hipLaunchKernelGGL((k_pf_final<NQ>), dim3(1), dim3(kPfFinalThreads), 0, st, in, count, out);
+}
+
+constexpr int pf_threads(bool greeks) { return greeks ? 256 : 64; }
+
+// Waves of k_pf_rows for n options (VW-option groups plus the odd tail).
+constexpr int64_t pf_waves(int64_t n, int vw) {
+    const int64_t items = vw == 2 ? n / 2 + (n & 1) : n;
+    int64_t waves = (items + 64 * kPfRows - 1) / (64 * kPfRows);
+    return waves < 1 ? 1 : waves;
+}
+
+// Doubles of scratch one call needs: the partials (whole blocks of waves) and
+// every level of chunk sums.
+constexpr int64_t pf_scratch_doubles(int64_t n, int vw, bool greeks) {
+    const int nq = greeks ? 5 : 1;
+    const int64_t wpb = pf_threads(greeks) / 64;
+    const int64_t waves = (pf_waves(n, vw) + wpb - 1) / wpb * wpb;
+    int64_t room = waves, level = waves;
+    while (level > kPfFinalMax) {
+        level = (level + kPfChunk - 1) / kPfChunk;
+        room += level;
+    }
+    return room * nq;
 }
 
 template <int VW, bool kGreeks>
-hipError_t launch_pf(const PfArgs& base, double* sums, hipStream_t st) {
+hipError_t launch_pf(const PfArgs& a, double* sums, hipStream_t st) {
     constexpr int NQ = kGreeks ? 5 : 1;
-    constexpr int NT = kGreeks ? 256 : 64;
-    // work-items: one per group, plus one for the last option of an odd book
-    const int64_t items = VW == 2 ? base.n / 2 + (base.n & 1) : base.n;
-    int64_t nb = (items + NT - 1) / NT;
-    if (nb < 1) nb = 1;
+    constexpr int NT = pf_threads(kGreeks);
+    constexpr int64_t wpb = NT / 64;
+    const int64_t waves = pf_waves(a.n, VW);
+    const int64_t nb = (waves + wpb - 1) / wpb;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    const int64_t nwaves = nb * (NT / 64);
-    // partials, then room for every level of chunk sums (each level is at most
-    // 1/kPfChunk of the one before)
-    const int64_t room = nwaves + nwaves / kPfChunk + 16;
-    PfArgs a = base;
-    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&a.partial),
-                                  (size_t)room * NQ * sizeof(double), st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_pf_waves<VW, kGreeks, NT>), dim3((unsigned)nb), dim3(NT), 0, st, a);
-    reduce_partials<NQ>(a.partial, nwaves, a.partial + nwaves * NQ, sums, st);
-    e = hipGetLastError();
-    const hipError_t f = hipFreeAsync(a.partial, st);
-    return e != hipSuccess ? e : f;
+    // LDS reserved per block caps the waves resident on a CU (none is used)
+    const int lw = config().lds_per_wave.load(std::memory_order_relaxed);
+    const size_t lds =
+        std::min<size_t>((size_t)(lw < 0 ? pf_lds_per_wave(kGreeks) : lw) * wpb, 65536);
+    if (a.n < (int64_t(1) << 30))
+        hipLaunchKernelGGL((k_pf_rows<VW, kGreeks, NT, uint32_t>), dim3((unsigned)nb), dim3(NT),
+                           lds, st, a);
+    else
+        hipLaunchKernelGGL((k_pf_rows<VW, kGreeks, NT, int64_t>), dim3((unsigned)nb), dim3(NT),
+                           lds, st, a);
+    reduce_partials<NQ>(a.partial, nb * wpb, a.partial + nb * wpb * NQ, sums, st);
+    return hipGetLastError();
+}
+
+bool pf_vec2(const PfArgs& a) {
+    auto al = [](const void* p, uintptr_t b) { return (reinterpret_cast<uintptr_t>(p) & (b - 1)) == 0; };
+    return al(a.in.s, 8) && al(a.in.k, 8) && al(a.in.r, 8) && al(a.in.v, 8) && al(a.in.t, 8) &&
+           al(a.type, 2) && (a.w == nullptr || al(a.w, 8));
+}
+
+hipError_t launch_pf_any(const PfArgs& a, bool greeks, double* sums, hipStream_t st) {
+    if (pf_vec2(a)) return greeks ? launch_pf<2, true>(a, sums, st) : launch_pf<2, false>(a, sums, st);
+    return greeks ? launch_pf<1, true>(a, sums, st) : launch_pf<1, false>(a, sums, st);
 }
 
 }  // namespace
 
 hipError_t launch_portfolio_f32(SoaIn<float> in, const uint8_t* type, const float* w, int64_t n,
                                 bool greeks, double* sums, hipStream_t st) {
-    auto al = [](const void* p, uintptr_t b) { return (reinterpret_cast<uintptr_t>(p) & (b - 1)) == 0; };
-    const bool vec2 = al(in.s, 8) && al(in.k, 8) && al(in.r, 8) && al(in.v, 8) && al(in.t, 8) &&
-                      al(type, 2) && (w == nullptr || al(w, 8));
-    const PfArgs a{in, type, w, n, nullptr};
-    if (vec2) return greeks ? launch_pf<2, true>(a, sums, st) : launch_pf<2, false>(a, sums, st);
-    return greeks ? launch_pf<1, true>(a, sums, st) : launch_pf<1, false>(a, sums, st);
+    PfArgs a{in, type, w, n, nullptr};
+    const int64_t need = pf_scratch_doubles(n, pf_vec2(a) ? 2 : 1, greeks);
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&a.partial),
+                                  (size_t)need * sizeof(double), st);
+    if (e != hipSuccess) return e;
+    e = launch_pf_any(a, greeks, sums, st);
+    const hipError_t f = hipFreeAsync(a.partial, st);
+    return e != hipSuccess ? e : f;
 }
 
 }  // namespace bs
