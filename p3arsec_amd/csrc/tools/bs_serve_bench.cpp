@@ -12,7 +12,10 @@
 // Latency = scheduled arrival -> completion observed by the dispatcher (the
 // request's outputs are resident in HBM and readable), both on
 // CLOCK_MONOTONIC, kept in a log-binned histogram (percentiles within
-// 0.12%).  Prints one JSON object.
+// 0.12%).  With --verify-every K, every K-th completed request's outputs are
+// copied back and compared bit for bit with the same book priced alone by
+// the batch entry points (bs_price_f32, bs_price_greeks_f32).  Prints one JSON
+// object.
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -49,7 +52,7 @@ void die(const char* what, int rc) {
 
 struct Opts {
     double rate = 20000.0, duration_s = 5.0, greeks_frac = 0.3, budget_gb = 32.0;
-    int books = 256, inflight = 2048, use_graph = 0, lane_options = 0;
+    int books = 256, inflight = 2048, use_graph = 0, lane_options = 0, verify_every = 0;
     int64_t max_book = 1 << 20, small = 65536;
     uint64_t seed = 20261207;
     bool saturate = false;
@@ -57,6 +60,23 @@ struct Opts {
 
 struct Book {
     int64_t id, n;
+};
+
+// A request's tag: its arrival (ns after the run's start) in the high bits,
+// its book's catalog index and its kind below, so a completion names all three.
+constexpr int kTagBookBits = 19;
+int64_t make_tag(int64_t arrival_rel, int book, int kind) {
+    return (arrival_rel << (kTagBookBits + 1)) | ((int64_t)book << 1) | (kind == BS_REQ_GREEKS);
+}
+int64_t tag_arrival(int64_t tag) { return tag >> (kTagBookBits + 1); }
+int tag_book(int64_t tag) { return (int)((tag >> 1) & ((1 << kTagBookBits) - 1)); }
+bool tag_greeks(int64_t tag) { return tag & 1; }
+
+// One book priced alone by the batch entry points: the outputs every serving
+// result of that book must equal bit for bit.
+struct Alone {
+    std::vector<float> price;
+    std::vector<float> greeks;   // 5 x n: price, delta, vega, rho, theta
 };
 
 // Latency histogram: 1000 log-spaced bins per decade from 100 ns up; a
@@ -111,6 +131,7 @@ int main(int argc, char** argv) {
         else if ((v = val("--seed"))) o.seed = strtoull(v, nullptr, 10);
         else if ((v = val("--use-graph"))) o.use_graph = atoi(v);
         else if ((v = val("--small-options"))) o.lane_options = atoi(v);
+        else if ((v = val("--verify-every"))) o.verify_every = atoi(v);
         else if (strcmp(argv[i], "--saturate") == 0) o.saturate = true;
         else {
             fprintf(stderr, "unknown argument %s\n", argv[i]);
@@ -128,8 +149,15 @@ int main(int argc, char** argv) {
     // catalog: log-uniform sizes, options drawn as in the batch benchmark
     std::mt19937_64 rng(o.seed);
     std::uniform_real_distribution<double> U(0.0, 1.0);
+    if (o.books > (1 << kTagBookBits)) {
+        fprintf(stderr, "--books is at most %d\n", 1 << kTagBookBits);
+        return 2;
+    }
     std::vector<Book> books;
+    std::vector<Alone> alone;
     int64_t catalog_options = 0;
+    float* dev = nullptr;   // scratch for the alone pricing: 6 inputs + 5 outputs of the largest book
+    if (o.verify_every > 0) CHECK(bs_malloc((void**)&dev, (size_t)o.max_book * 11 * sizeof(float)));
     for (int b = 0; b < o.books; ++b) {
         const int64_t n = std::max<int64_t>(
             1, (int64_t)std::floor(std::exp(U(rng) * std::log((double)o.max_book + 1.0))));
@@ -148,7 +176,31 @@ int main(int argc, char** argv) {
                                &id));
         books.push_back({id, n});
         catalog_options += n;
+        if (o.verify_every > 0) {
+            const size_t nb = (size_t)n * sizeof(float);
+            float* in[5] = {dev, dev + o.max_book, dev + 2 * o.max_book, dev + 3 * o.max_book,
+                            dev + 4 * o.max_book};
+            uint8_t* dty = (uint8_t*)(dev + 5 * o.max_book);
+            float* out = dev + 6 * o.max_book;
+            const float* src[5] = {s.data(), k.data(), r.data(), v.data(), t.data()};
+            for (int j = 0; j < 5; ++j) CHECK(bs_memcpy_h2d(in[j], src[j], nb, nullptr));
+            CHECK(bs_memcpy_h2d(dty, ty.data(), (size_t)n, nullptr));
+            Alone a;
+            a.price.resize(n);
+            a.greeks.resize(5 * (size_t)n);
+            CHECK(bs_price_f32(in[0], in[1], in[2], in[3], in[4], dty, out, n, nullptr));
+            CHECK(bs_memcpy_d2h(a.price.data(), out, nb, nullptr));
+            CHECK(bs_price_greeks_f32(in[0], in[1], in[2], in[3], in[4], dty, out,
+                                      out + o.max_book, out + 2 * o.max_book,
+                                      out + 3 * o.max_book, out + 4 * o.max_book, n, nullptr));
+            for (int j = 0; j < 5; ++j)
+                CHECK(bs_memcpy_d2h(a.greeks.data() + (size_t)j * n, out + (size_t)j * o.max_book,
+                                    nb, nullptr));
+            CHECK(bs_device_sync());
+            alone.push_back(std::move(a));
+        }
     }
+    if (dev) CHECK(bs_free(dev));
 
     // Completions come from the context's completion queue (bs_reap) in the
     // order requests finish; each is released right away.  Each request's tag
@@ -159,10 +211,14 @@ int main(int argc, char** argv) {
     int64_t options_done = 0, requests_done = 0;
     std::atomic<int64_t> outstanding{0};
     int64_t c_wait = 0, c_work = 0, s_wait = 0, s_work = 0;   // client thread time, ns
+    int64_t verified[2][2] = {{0, 0}, {0, 0}};   // [greeks][n > 64K]
+    int64_t mismatched = 0;
+    const int64_t t0 = now_ns();
 
     std::thread completer([&] {
         std::vector<bs_completion> cq(8192);
         std::vector<bs_ticket_t> rel;
+        std::vector<float> got(5 * (size_t)o.max_book);
         for (;;) {
             int64_t k = 0;
             const int64_t w0 = now_ns();
@@ -177,7 +233,19 @@ int main(int argc, char** argv) {
             for (int64_t i = 0; i < k; ++i) {
                 const bs_completion& c = cq[i];
                 if (c.state != BS_TICKET_DONE) die("request", c.status);
-                const double ns = (double)(c.done_ns - c.user);
+                const double ns = (double)(c.done_ns - (t0 + tag_arrival(c.user)));
+                if (o.verify_every > 0 && requests_done % o.verify_every == 0) {
+                    const Alone& a = alone[tag_book(c.user)];
+                    const bool g = tag_greeks(c.user);
+                    const size_t n = (size_t)c.n;
+                    float* p[5] = {got.data(), got.data() + n, got.data() + 2 * n,
+                                   got.data() + 3 * n, got.data() + 4 * n};
+                    if (g) CHECK(bs_result_copy(ctx, c.ticket, p[0], p[1], p[2], p[3], p[4]));
+                    else CHECK(bs_result_copy(ctx, c.ticket, p[0], nullptr, nullptr, nullptr, nullptr));
+                    const float* want = g ? a.greeks.data() : a.price.data();
+                    if (memcmp(got.data(), want, (g ? 5 : 1) * n * sizeof(float)) != 0) ++mismatched;
+                    ++verified[g][c.n > o.small];
+                }
                 lat_all.add(ns);
                 if (c.n <= o.small) lat_small.add(ns);
                 options_done += c.n;
@@ -192,7 +260,6 @@ int main(int argc, char** argv) {
 
     std::exponential_distribution<double> gap(o.rate);
     std::uniform_int_distribution<int> pick(0, (int)books.size() - 1);
-    const int64_t t0 = now_ns();
     const int64_t t_end = t0 + (int64_t)(o.duration_s * 1e9);
     double next = (double)t0 + gap(rng) * 1e9;   // the next scheduled arrival (open loop)
     int64_t submitted = 0;
@@ -201,9 +268,11 @@ int main(int argc, char** argv) {
     std::vector<int64_t> tags;
     std::vector<bs_ticket_t> tks;
     auto add = [&](int64_t arrival) {
-        ids.push_back(books[pick(rng)].id);
-        kinds.push_back(U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE);
-        tags.push_back(arrival);
+        const int b = pick(rng);
+        const int kind = U(rng) < o.greeks_frac ? BS_REQ_GREEKS : BS_REQ_PRICE;
+        ids.push_back(books[b].id);
+        kinds.push_back(kind);
+        tags.push_back(make_tag(std::max<int64_t>(0, arrival - t0), b, kind));
     };
     for (;;) {
         ids.clear();
@@ -258,7 +327,9 @@ int main(int argc, char** argv) {
            "\"dispatch_ms\": {\"admit\": %.1f, \"launch\": %.1f, \"retire\": %.1f, "
            "\"gpu_idle\": %.1f, \"idle_launches\": %lld}, "
            "\"client_ms\": {\"reap_wait\": %.1f, \"reap_work\": %.1f, \"submit_wait\": %.1f, "
-           "\"submit_work\": %.1f}}\n",
+           "\"submit_work\": %.1f}, \"verify_every\": %d, \"verified\": {\"price_le64k\": %lld, "
+           "\"price_gt64k\": %lld, \"greeks_le64k\": %lld, \"greeks_gt64k\": %lld}, "
+           "\"mismatched\": %lld}\n",
            o.saturate ? "closed-loop saturation" : "open-loop poisson", o.saturate ? 0.0 : o.rate,
            wall, (long long)requests_done, (long long)options_done, (double)options_done / wall,
            lat_all.pct_ms(0.50), lat_all.pct_ms(0.99), lat_small.pct_ms(0.50), lat_small.pct_ms(0.99),
@@ -272,6 +343,8 @@ int main(int argc, char** argv) {
            st.launches ? (double)st.options_priced / (double)st.launches : 0.0,
            st.dispatch_admit_ns * 1e-6, st.dispatch_launch_ns * 1e-6, st.dispatch_retire_ns * 1e-6,
            st.gpu_idle_ns * 1e-6, (long long)st.idle_launches, c_wait * 1e-6, c_work * 1e-6,
-           s_wait * 1e-6, s_work * 1e-6);
+           s_wait * 1e-6, s_work * 1e-6, o.verify_every, (long long)verified[0][0],
+           (long long)verified[0][1], (long long)verified[1][0], (long long)verified[1][1],
+           (long long)mismatched);
     return 0;
 }
