@@ -155,6 +155,19 @@ int bs_portfolio_f32(const float *s, const float *k, const float *r, const float
                      const float *t, const uint8_t *otype, const float *weights, int64_t n,
                      int greeks, double *sums, bs_stream_t stream);
 
+/* bs_portfolio_f32 that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (work under HIP graph
+ * capture, serving loops and harnesses that forbid hipMalloc).  The same
+ * arguments, checks and sums, bit for bit, for n <= 2^25 options
+ * (BS_ERR_INVALID past that); the partial sums live in a fixed device array
+ * of the library (2.6 MB on each device it runs on, held until the process
+ * ends) instead of stream-ordered memory.  Calls that overlap on one device
+ * share that array and race: issue them on one stream, or let one finish
+ * before the next starts. */
+int bs_portfolio_f32_noalloc(const float *s, const float *k, const float *r, const float *v,
+                             const float *t, const uint8_t *otype, const float *weights,
+                             int64_t n, int greeks, double *sums, bs_stream_t stream);
+
 /* AoS -> SoA repack on the device (BS:747-767).  `aos` points at n records
  * laid out as the reference's OptionData (BS:89-100): float s, strike, r,
  * divq, v, t; char OptionType; float divs, DGrefval -- 36 bytes each.  The

@@ -110,6 +110,7 @@ _SIGS = {
     "bs_stream_destroy": ([_vp], _i32),
     "bs_stream_sync": ([_vp], _i32),
     "bs_portfolio_f32": ([_vp] * 7 + [_i64, _i32, _vp, _vp], _i32),
+    "bs_portfolio_f32_noalloc": ([_vp] * 7 + [_i64, _i32, _vp, _vp], _i32),
     "bs_stream_wait_event": ([_vp, _vp], _i32),
     "bs_device_sync": ([], _i32),
     "bs_event_create": ([ctypes.POINTER(_vp)], _i32),

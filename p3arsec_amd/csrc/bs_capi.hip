@@ -405,6 +405,24 @@ int bs_portfolio_f32(const float* s, const float* k, const float* r, const float
                 "bs_portfolio_f32");
 }
 
+int bs_portfolio_f32_noalloc(const float* s, const float* k, const float* r, const float* v,
+                             const float* t, const uint8_t* otype, const float* weights,
+                             int64_t n, int greeks, double* sums, bs_stream_t stream) {
+    int rc;
+    if ((rc = check_n(n))) return rc;
+    if (n > kPfNoAllocMaxOptions)
+        return set_error(BS_ERR_INVALID,
+                         "bs_portfolio_f32_noalloc: n exceeds 2^25 options, the size its fixed "
+                         "scratch holds (bs_portfolio_f32 takes any n)");
+    if ((rc = check_device_range(sums, (greeks ? 5 : 1) * sizeof(double), "sums"))) return rc;
+    if (n > 0 && ((rc = check_soa(s, k, r, v, t, n)) ||
+                  (rc = check_device_range(otype, (size_t)n, "otype")) ||
+                  (weights && (rc = check_device_range(weights, (size_t)n * 4, "weights")))))
+        return rc;
+    return done(launch_portfolio_f32_noalloc({s, k, r, v, t}, otype, weights, n, greeks != 0,
+                                             sums, as_stream(stream)),
+                "bs_portfolio_f32_noalloc");
+}
 
 int bs_repack_aos_f32(const void* aos, int64_t n, float* s, float* k, float* r, float* v,
                       float* t, uint8_t* otype, bs_stream_t stream) {

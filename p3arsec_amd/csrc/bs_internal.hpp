@@ -82,6 +82,11 @@ hipError_t launch_cndf_f32(const float* x, float* out, int64_t n, hipStream_t st
 // Portfolio sums (bs_portfolio.hip); w may be NULL.
 hipError_t launch_portfolio_f32(SoaIn<float> in, const uint8_t* type, const float* w, int64_t n,
                                 bool greeks, double* sums, hipStream_t st);
+// The same sums with the device's fixed scratch (bs_portfolio.hip):
+// n <= kPfNoAllocMaxOptions; no allocation.
+constexpr int64_t kPfNoAllocMaxOptions = int64_t(1) << 25;
+hipError_t launch_portfolio_f32_noalloc(SoaIn<float> in, const uint8_t* type, const float* w,
+                                        int64_t n, bool greeks, double* sums, hipStream_t st);
 hipError_t launch_cndf_f64(const double* x, double* out, int64_t n, hipStream_t st);
 hipError_t launch_repack_aos(const void* aos, int64_t n, float* s, float* k, float* r,
                              float* v, float* t, uint8_t* type, hipStream_t st);

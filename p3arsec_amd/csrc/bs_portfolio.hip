@@ -18,12 +18,15 @@
 // contiguous chunks of 2048.  Every grid depends only on n, so the sums are
 // bit-reproducible for a given book.
 //
-// Scratch (the partials and the chunk sums) comes from stream-ordered memory
-// (hipMallocAsync/hipFreeAsync), so calls on different streams never share it.
+// Scratch (the partials and the chunk sums): bs_portfolio_f32 takes it from
+// stream-ordered memory (hipMallocAsync/hipFreeAsync), so calls on different
+// streams never share it; bs_portfolio_f32_noalloc uses a fixed device array
+// of this file (g_pf_scratch) and allocates nothing.
 //
 // Traffic: 21 bytes per option read (25 with weights), 8 or 40 bytes written
 // in all, plus 8 or 40 bytes of partials per kPfRows x 128 options.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "bs_internal.hpp"
@@ -273,7 +276,53 @@ hipError_t launch_pf_any(const PfArgs& a, bool greeks, double* sums, hipStream_t
     return greeks ? launch_pf<1, true>(a, sums, st) : launch_pf<1, false>(a, sums, st);
 }
 
+// ---- the form that allocates nothing -----------------------------------------
+// Its scratch is this fixed device array, sized for the largest call it takes
+// (kPfNoAllocMaxOptions options with Greeks on unaligned arrays, the most
+// partials any call of that size needs): 2.6 MB, held on every device the
+// library runs on until the process ends.  In host code the array's name is
+// not a device address: its address on each device comes from
+// hipGetSymbolAddress, resolved once per device with that device current, and
+// only that address reaches a kernel.
+constexpr int64_t kPfScratchDoubles = pf_scratch_doubles(kPfNoAllocMaxOptions, 1, true);
+static_assert(kPfScratchDoubles >= pf_scratch_doubles(kPfNoAllocMaxOptions, 2, true) &&
+                  kPfScratchDoubles >= pf_scratch_doubles(kPfNoAllocMaxOptions, 1, false),
+              "scratch sized for the worst form");
+__device__ __align__(256) double g_pf_scratch[kPfScratchDoubles];
+
+constexpr int kPfMaxDevices = 64;
+std::mutex g_pf_scratch_mu;
+double* g_pf_scratch_ptr[kPfMaxDevices] = {};
+
+hipError_t pf_scratch_of(int dev, double** out) {
+    if (dev < 0 || dev >= kPfMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_pf_scratch_mu);
+    if (!g_pf_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_pf_scratch));
+        if (e != hipSuccess) return e;
+        g_pf_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_pf_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
+
+// bs_portfolio_f32 with the device's fixed scratch: no allocation.  The caller
+// has checked n <= kPfNoAllocMaxOptions.  Calls that overlap on one device
+// share the scratch and race.
+hipError_t launch_portfolio_f32_noalloc(SoaIn<float> in, const uint8_t* type, const float* w,
+                                        int64_t n, bool greeks, double* sums, hipStream_t st) {
+    if (n > kPfNoAllocMaxOptions) return hipErrorInvalidValue;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    PfArgs a{in, type, w, n, nullptr};
+    e = pf_scratch_of(dev, &a.partial);
+    if (e != hipSuccess) return e;
+    return launch_pf_any(a, greeks, sums, st);
+}
 
 hipError_t launch_portfolio_f32(SoaIn<float> in, const uint8_t* type, const float* w, int64_t n,
                                 bool greeks, double* sums, hipStream_t st) {

@@ -64,6 +64,9 @@ def test_argument_errors_before_device_work():
     assert L.bs_price_f32(*nul, 16, None) == -1           # NULL arrays
     assert b"NULL" in L.bs_last_error()
     assert L.bs_price_greeks_f32(*([None] * 11), 0, None) == 0
+    # the allocation-free portfolio form holds books of at most 2^25 options
+    assert L.bs_portfolio_f32_noalloc(*nul, (1 << 25) + 1, 0, None, None) == -1
+    assert b"2^25" in L.bs_last_error()
     assert L.bs_price_host_f32(*nul, 8, 1, 0) == -1
     host = np.zeros(16, np.float32)
     p = host.ctypes.data
