@@ -80,10 +80,10 @@ def measured_traffic(kernel: str, n: int):
 
 LAUNCH_CPU_US = {}
 # Streams the passes of each map alternate over, the faster schedule measured
-# on MI355X (10M options): price 37.9 us per pass on two streams vs 40.5 on
-# one; FP64 109.5 vs 113; the write-heavy Greeks map 70.5 us on one stream vs
-# 73.3 on two.
-STREAMS = {"price": 2, "fast": 2, "greeks": 1, "fp64": 2}
+# on MI355X (10M options, 300 passes): price 37.25 / 37.52 us per pass on two
+# streams vs 37.98 / 39.41 on one (two boxes); Greeks 67.22 vs 67.98; FP64
+# 85 vs 89.
+STREAMS = {"price": 2, "fast": 2, "greeks": 2, "fp64": 2}
 
 
 def reap_children() -> int:
@@ -300,12 +300,19 @@ def reference_cpu(book, threads: int, port_prices, repeats: int = 1) -> dict:
 
 
 def cpu_model() -> str:
-    """lscpu's model name (the full string), else /proc/cpuinfo's."""
+    """lscpu's model name (the full string) with its CPU family, model and
+    stepping numbers (a virtual machine may report only "AMD EPYC"; the numbers
+    still name the generation), else /proc/cpuinfo's model name."""
     try:
         p = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10)
+        f = {}
         for ln in p.stdout.splitlines():
-            if ln.startswith("Model name:"):
-                return ln.split(":", 1)[1].strip()
+            k, _, v = ln.partition(":")
+            f.setdefault(k.strip(), v.strip())
+        if f.get("Model name"):
+            ids = ", ".join(f"{k.lower()} {f[k]}" for k in ("CPU family", "Model", "Stepping")
+                            if f.get(k))
+            return f"{f['Model name']} ({ids})" if ids else f["Model name"]
     except (OSError, subprocess.SubprocessError):
         pass
     try:

@@ -71,11 +71,11 @@ enum {
     BS_CFG_VECTOR_WIDTH = 4,   /* options per work-item: 0 (default), 2, 4   */
     BS_CFG_NONTEMPORAL = 5,    /* 1 (default): non-temporal loads and stores  */
     BS_CFG_PIPELINE = 6,       /* -1 (default), 0, 1: software pipelining     */
-    BS_CFG_LDS_PER_WAVE = 7    /* LDS bytes reserved per wave of a streaming
-                                  block (at most 64 KB per block), capping
-                                  the waves resident on a CU: -1 (default:
-                                  each kernel's measured best), 0 (none) or
-                                  1..65536                                 */
+    BS_CFG_LDS_PER_WAVE = 7    /* LDS bytes reserved per wave of a block of the
+                                  batch maps and portfolio sums (at most 64 KB
+                                  per block), capping the waves resident on a
+                                  CU: -1 (default: each kernel's measured
+                                  best), 0 (none) or 1..65536               */
 };
 int bs_config_set(int key, int64_t value);
 int bs_config_get(int key, int64_t *value);

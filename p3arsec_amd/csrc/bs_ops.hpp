@@ -12,15 +12,17 @@
 // per SIMD = 64 VGPRs), kBlocks (default grid cap, 0 = one group per
 // work-item) and kPipe (default loop form); the defaults and each launcher's
 // preferred width are the fastest forms measured on MI355X for 10M options.
-// Defaults measured on MI355X, 10M options, warm clocks, passes on one / two
-// streams: price 64 threads, 6 KB of LDS per wave 39.2 / 38.1 us (no LDS
-// reserved: 40.8 / 38.2; 256 threads: 43.4 / 42.2); Greeks 64 threads, 11 KB
-// per wave 68.0 / 67.0 us (no LDS: 74.1 / 74.5; 256 threads, no LDS: 71.2 /
-// 74.7); int32-type price 64 threads 44.9 us, FP64 64 threads pipelined
-// 108 us.  Small blocks help: one-wave workgroups retire and refill
-// independently, so the tail of the grid is short.  Fewer resident waves help
-// the write-heavy Greeks stream most: with 14 waves per CU instead of 32 its
-// five output streams keep fewer DRAM pages open at once.
+// Defaults measured on MI355X, 10M options, warm clocks, 300 passes on one /
+// two streams, with the agent-scope stores below: price (Ref32) 64 threads,
+// no LDS reserved 37.98 / 37.25 us (6 KB per wave: 38.64 / 37.90; 7 KB:
+// 40.1 / 39.3); Fast32 64 threads, 6 KB 37.56 / 36.95 us (none: 39.06 /
+// 37.80); Greeks 64 threads, 11 KB per wave 67.98 / 67.22 us (none: 74.16 /
+// 73.79; 9 KB: 68.18 / 68.52; 12 KB: 70.82 / 69.59); int32-type price 64
+// threads 44.9 us, FP64 64 threads pipelined 85 us.  Small blocks help:
+// one-wave workgroups retire and refill independently, so the tail of the
+// grid is short.  Fewer resident waves help the write-heavy Greeks stream
+// most: with 14 waves per CU instead of 32 its five output streams keep fewer
+// DRAM pages open at once.
 #pragma once
 #include <type_traits>
 
@@ -41,10 +43,22 @@ __device__ __forceinline__ T ld(const T* p) {
     if constexpr (NT) return __builtin_nontemporal_load(p);
     else return *p;
 }
+// Streaming stores.  An 8-byte value goes out as a relaxed agent-scope atomic
+// store: the same global_store_dwordx2, with the memory model's agent-scope
+// cache policy (sc1), which on MI355X streams the outputs faster than the
+// non-temporal hint (10M options: price map on one stream 38.9 -> 37.9 us,
+// Greeks 67.9 -> 67.1 us).  Other sizes keep the non-temporal store (the FP64
+// map's 16-byte stores split in two ran 4 us slower).
 template <bool NT, class T>
 __device__ __forceinline__ void st(T* p, T v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
+    if constexpr (NT && sizeof(T) == 8) {
+        __hip_atomic_store(reinterpret_cast<uint64_t*>(p), __builtin_bit_cast(uint64_t, v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if constexpr (NT) {
+        __builtin_nontemporal_store(v, p);
+    } else {
+        *p = v;
+    }
 }
 
 // Element g of an array.  I = int64_t: 64-bit index arithmetic.  I = uint32_t
@@ -70,7 +84,9 @@ template <class Math, int VW>
 struct PriceF32 {
     static constexpr int kVW = VW;
     static constexpr int kElemBytes = 4;   // widest array element
-    static constexpr int kLdsPerWave = 6144;   // 26 waves per CU
+    // Fast32 runs best at 26 waves per CU; Ref32's longer arithmetic needs
+    // every wave slot to hide its latency
+    static constexpr int kLdsPerWave = std::is_same<Math, Fast32>::value ? 6144 : 0;
     static constexpr int kWaves = 8;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
