@@ -67,7 +67,9 @@ int bs_version(void); /* 10000*major + 100*minor + patch */
 enum {
     BS_CFG_CHECK_POINTERS = 1, /* 1 (default): validate device pointers      */
     BS_CFG_MAX_BLOCKS = 2,     /* grid cap for streaming kernels, 0 = auto    */
-    BS_CFG_BLOCK_THREADS = 3,  /* streaming block size: 0 (default), 64..256 */
+    BS_CFG_BLOCK_THREADS = 3,  /* streaming block size: 0 (default), 64, 128
+                                  or 256; bs_ctx_create reads it for the
+                                  serving context's fused kernels          */
     BS_CFG_VECTOR_WIDTH = 4,   /* options per work-item: 0 (default), 2, 4   */
     BS_CFG_NONTEMPORAL = 5,    /* 1 (default): non-temporal loads and stores  */
     BS_CFG_PIPELINE = 6,       /* -1 (default), 0, 1: software pipelining     */

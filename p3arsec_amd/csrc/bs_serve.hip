@@ -42,7 +42,10 @@ namespace bs {
 namespace {
 
 constexpr int kVW = 2;                 // options per work-item (the batch map's width)
-constexpr int kTile = kThreads * kVW;  // options per tile: one block step
+// Threads per block of the fused kernels, chosen per context from
+// BS_CFG_BLOCK_THREADS (64, 128 or 256; 0 = kServeThreads); a tile is one
+// block step, threads x kVW options.
+constexpr int kServeThreads = 256;
 constexpr int kMaxDesc = 4096;         // requests per batch: 64 x 64 for find_desc
 constexpr int64_t kMaxGrid = int64_t(1) << 20;  // default section grid cap (blocks loop)
 constexpr int kStageBlocks = 32;
@@ -106,14 +109,14 @@ __device__ __forceinline__ GreeksF32<kVW> op_of(const Desc& x) {
             GreeksOut{x.out[0], x.out[1], x.out[2], x.out[3], x.out[4]}};
 }
 
-// Fused kernel of one section: a block prices tile t (kTile options of one
-// request, one kVW-group per work-item) for t = blockIdx.x, + gridDim.x, ...
+// Fused kernel of one section: a block prices tile t (kThr x kVW options of
+// one request, one kVW-group per work-item) for t = blockIdx.x, + gridDim.x, ...
 // The grid has one block per tile up to the context's cap (kMaxGrid unless
 // bs_serve_opts.grid_blocks sets it), in both launch forms.
 // Every access is bounded by the request's n.
 // The per-group code is the batch map's own (bs_ops.hpp).
-template <class Op, int kSection>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(waves_for<Op, false>())))
+template <class Op, int kSection, int kThr>
+__global__ __launch_bounds__(kThr) __attribute__((amdgpu_waves_per_eu(waves_for<Op, false>())))
 void k_serve(const uint8_t* __restrict__ slot) {
     const SlotHeader* h = reinterpret_cast<const SlotHeader*>(slot);
     const int np = h->ndesc[0], ng = h->ndesc[1];
@@ -126,7 +129,7 @@ void k_serve(const uint8_t* __restrict__ slot) {
         const Desc& x = d[find_desc(d, nd, t)];
         const Op op = op_of<Op>(x);
         const int64_t n = x.n;
-        const int64_t g = (t - x.tile_begin) * kThreads + threadIdx.x;
+        const int64_t g = (t - x.tile_begin) * kThr + threadIdx.x;
         if ((g + 1) * kVW <= n) {
             op.template finish<true>(g, op.template load<true>(g));
         } else {
@@ -137,6 +140,17 @@ void k_serve(const uint8_t* __restrict__ slot) {
 
 using ServePrice = PriceF32<Ref32, kVW>;
 using ServeGreeks = GreeksF32<kVW>;
+
+// The fused kernel of section sec with kThr threads per block.
+template <int kThr>
+const void* serve_fn(int sec) {
+    return sec == 0 ? reinterpret_cast<const void*>(&k_serve<ServePrice, 0, kThr>)
+                    : reinterpret_cast<const void*>(&k_serve<ServeGreeks, 1, kThr>);
+}
+const void* serve_fn(int threads, int sec) {
+    return threads == 64 ? serve_fn<64>(sec) : threads == 128 ? serve_fn<128>(sec)
+                                                                : serve_fn<256>(sec);
+}
 
 }  // namespace
 
@@ -153,6 +167,8 @@ struct HipExec {
 
     int device = 0;
     int use_graph = 0;
+    int threads = kServeThreads;                // fused kernels' block size
+    int64_t tile = kServeThreads * kVW;         // options per tile
     int64_t grid_cap = kMaxGrid;                // blocks per section at most
     char* arena = nullptr;
     hipStream_t stream = nullptr;               // capture stream
@@ -170,9 +186,12 @@ struct HipExec {
     hipStream_t slot_stream[kSlots] = {};        // the stream slot s was last launched on
     int64_t seq = 0, nlaunch = 0;
 
-    hipError_t init(int dev, size_t arena_bytes, int graph_on, int grid_blocks) {
+    hipError_t init(int dev, size_t arena_bytes, int graph_on, int grid_blocks, int block_threads) {
         device = dev;
         use_graph = graph_on;
+        if (block_threads == 64 || block_threads == 128 || block_threads == 256)
+            threads = block_threads;
+        tile = (int64_t)threads * kVW;
         hipError_t e = hipSetDevice(device);
         if (e == hipSuccess) e = hipMalloc((void**)&arena, arena_bytes);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
@@ -215,16 +234,14 @@ struct HipExec {
         hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, stream,
                            reinterpret_cast<const uint4*>(host_slot_dev[s]),
                            reinterpret_cast<uint4*>(dev_slot[s]));
-        hipLaunchKernelGGL((k_serve<ServePrice, 0>), dim3(1), dim3(kThreads), 0, stream,
-                           dev_slot[s]);
-        hipLaunchKernelGGL((k_serve<ServeGreeks, 1>), dim3(1), dim3(kThreads), 0, stream,
-                           dev_slot[s]);
+        section_args[s][0] = &dev_slot[s];
+        for (int sec = 0; sec < 2; ++sec)
+            (void)hipLaunchKernel(serve_fn(threads, sec), dim3(1), dim3(threads), section_args[s],
+                                  0, stream);
         e = hipStreamEndCapture(stream, &g);
         if (e != hipSuccess) return e;
         graph_src[s] = g;
-        section_args[s][0] = &dev_slot[s];
-        const void* fn[2] = {reinterpret_cast<const void*>(&k_serve<ServePrice, 0>),
-                             reinterpret_cast<const void*>(&k_serve<ServeGreeks, 1>)};
+        const void* fn[2] = {serve_fn(threads, 0), serve_fn(threads, 1)};
         hipGraphNode_t nodes[3];
         size_t count = 3;
         e = hipGraphGetNodes(g, nodes, &count);
@@ -245,10 +262,9 @@ struct HipExec {
     hipError_t set_section_grid(int s, int sec, unsigned blocks) {
         if (section_grid[s][sec] == blocks) return hipSuccess;
         hipKernelNodeParams p{};
-        p.func = sec == 0 ? reinterpret_cast<void*>(&k_serve<ServePrice, 0>)
-                          : reinterpret_cast<void*>(&k_serve<ServeGreeks, 1>);
+        p.func = const_cast<void*>(serve_fn(threads, sec));
         p.gridDim = dim3(blocks);
-        p.blockDim = dim3(kThreads);
+        p.blockDim = dim3(threads);
         p.sharedMemBytes = 0;
         p.kernelParams = section_args[s];
         p.extra = nullptr;
@@ -280,7 +296,7 @@ struct HipExec {
             x.tile_begin = tiles[sec];
             x.kind = r.kind;
             x.pad = 0;
-            tiles[sec] += (r.n + kTile - 1) / kTile;
+            tiles[sec] += (r.n + tile - 1) / tile;
         }
         h->ndesc[0] = nd[0];
         h->ndesc[1] = nd[1];
@@ -313,15 +329,13 @@ struct HipExec {
             hipLaunchKernelGGL(k_stage, dim3(kStageBlocks), dim3(kThreads), 0, st,
                                reinterpret_cast<const uint4*>(host_slot_dev[s]),
                                reinterpret_cast<uint4*>(dev_slot[s]));
-            if (h->tiles[0] > 0)
-                hipLaunchKernelGGL((k_serve<ServePrice, 0>),
-                                   dim3((unsigned)std::min(h->tiles[0], grid_cap)), dim3(kThreads),
-                                   0, st, dev_slot[s]);
-            if (h->tiles[1] > 0)
-                hipLaunchKernelGGL((k_serve<ServeGreeks, 1>),
-                                   dim3((unsigned)std::min(h->tiles[1], grid_cap)), dim3(kThreads),
-                                   0, st, dev_slot[s]);
             e = hipGetLastError();
+            void* args[1] = {&dev_slot[s]};
+            for (int sec = 0; sec < 2 && e == hipSuccess; ++sec)
+                if (h->tiles[sec] > 0)
+                    e = hipLaunchKernel(serve_fn(threads, sec),
+                                        dim3((unsigned)std::min(h->tiles[sec], grid_cap)),
+                                        dim3(threads), args, 0, st);
         }
         if (e == hipSuccess) e = hipEventRecord(ev[s], st);
         if (e != hipSuccess) *err = hipGetErrorString(e);
@@ -386,6 +400,7 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
         lim.max_batch_options = int64_t(1) << 31;
         lim.small_options = 65536;
         int use_graph = 0, grid_blocks = 0;
+        const int block_threads = config().block_threads;
         if (opts) {
             if (opts->small_request_options != 0)
                 lim.small_options = opts->small_request_options < 0 ? 0 : opts->small_request_options;
@@ -396,7 +411,7 @@ int bs_ctx_create(int device, size_t hbm_budget_bytes, const bs_serve_opts* opts
             grid_blocks = opts->grid_blocks;
         }
         auto* c = new bs_ctx_s();
-        e = c->exec.init(device, hbm_budget_bytes, use_graph, grid_blocks);
+        e = c->exec.init(device, hbm_budget_bytes, use_graph, grid_blocks, block_threads);
         if (e != hipSuccess) {
             const int rc = hip_error(e, "bs_ctx_create");
             c->exec.release();

@@ -53,6 +53,7 @@ void die(const char* what, int rc) {
 struct Opts {
     double rate = 20000.0, duration_s = 5.0, greeks_frac = 0.3, budget_gb = 32.0;
     int books = 256, inflight = 2048, use_graph = 0, lane_options = 0, verify_every = 0;
+    int block_threads = 0;
     int64_t max_book = 1 << 20, small = 65536;
     uint64_t seed = 20261207;
     bool saturate = false;
@@ -132,6 +133,7 @@ int main(int argc, char** argv) {
         else if ((v = val("--use-graph"))) o.use_graph = atoi(v);
         else if ((v = val("--small-options"))) o.lane_options = atoi(v);
         else if ((v = val("--verify-every"))) o.verify_every = atoi(v);
+        else if ((v = val("--block-threads"))) o.block_threads = atoi(v);
         else if (strcmp(argv[i], "--saturate") == 0) o.saturate = true;
         else {
             fprintf(stderr, "unknown argument %s\n", argv[i]);
@@ -139,6 +141,8 @@ int main(int argc, char** argv) {
         }
     }
 
+    // the serving kernels' block size is read when the context is created
+    if (o.block_threads) CHECK(bs_config_set(BS_CFG_BLOCK_THREADS, o.block_threads));
     bs_serve_opts so{};
     so.use_graph = o.use_graph;
     so.completion_queue = 1;
@@ -321,7 +325,8 @@ int main(int argc, char** argv) {
            "\"p50_ms_le64k\": %.4f, \"p99_ms_le64k\": %.4f, \"max_ms\": %.4f, "
            "\"launches\": %lld, \"mean_requests_per_launch\": %.2f, \"books\": %d, "
            "\"catalog_options\": %lld, \"greeks_frac\": %.2f, \"arena_peak_bytes\": %lld, "
-           "\"admission_waits\": %lld, \"use_graph\": %d, \"small_request_options\": %d, "
+           "\"admission_waits\": %lld, \"use_graph\": %d, \"block_threads\": %d, "
+           "\"small_request_options\": %d, "
            "\"mean_queue_us\": %.2f, "
            "\"mean_batch_us\": %.2f, \"mean_options_per_launch\": %.0f, "
            "\"dispatch_ms\": {\"admit\": %.1f, \"launch\": %.1f, \"retire\": %.1f, "
@@ -337,7 +342,7 @@ int main(int argc, char** argv) {
            (long long)st.launches,
            st.launches ? (double)requests_done / (double)st.launches : 0.0, o.books,
            (long long)catalog_options, o.greeks_frac, (long long)st.arena_peak,
-           (long long)st.admission_waits, o.use_graph, o.lane_options,
+           (long long)st.admission_waits, o.use_graph, o.block_threads, o.lane_options,
            st.requests_done ? (double)st.queue_ns_total * 1e-3 / (double)st.requests_done : 0.0,
            st.launches ? (double)st.batch_ns_total * 1e-3 / (double)st.launches : 0.0,
            st.launches ? (double)st.options_priced / (double)st.launches : 0.0,

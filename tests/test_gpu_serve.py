@@ -111,6 +111,42 @@ def test_alone_and_in_the_mix(gpu, use_graph, small):
         ctx.destroy()
 
 
+@pytest.mark.parametrize("threads,use_graph,grid", [(64, 0, 0), (64, 1, 0), (128, 0, 0),
+                                                    (128, 1, 5), (64, 0, 5)])
+def test_block_threads_alone_and_in_the_mix(gpu, threads, use_graph, grid):
+    """BS_CFG_BLOCK_THREADS at context creation sets the fused kernels' block
+    (a tile is threads x 2 options): results stay bit-identical to the batch
+    maps priced with the default launch shape, alone and in a 120-request mix,
+    in both launch forms and with a capped grid (blocks loop over tiles)."""
+    rng = np.random.default_rng(threads + grid)
+    sizes = [1, 2, 127, 128, 129, 255, 256, 257, 4097, 70_001, 262_144]
+    bks = [books.make_book(n, books.SEED_BASE + 90 + i) for i, n in enumerate(sizes)]
+    ref = {(i, k): standalone(gpu, bks[i], k) for i in range(len(bks)) for k in (1, 2)}
+    gpu.call("bs_config_set", gpu.CFG_BLOCK_THREADS, threads)
+    try:
+        ctx = Ctx(gpu, 512 << 20, use_graph=use_graph, grid=grid)
+    finally:
+        gpu.call("bs_config_set", gpu.CFG_BLOCK_THREADS, 0)
+    try:
+        ids = [ctx.register(b) for b in bks]
+        reqs = [(i, k) for i in range(len(bks)) for k in (1, 2)]
+        for i, k in reqs:                                   # alone
+            t = ctx.submit(ids[i], k)
+            assert ctx.wait(t) == 0
+            assert same_bits(ctx.result(t, bks[i].n, k), ref[(i, k)]), (threads, i, k)
+            gpu.call("bs_result_release", ctx.h, t)
+        reqs = [(int(rng.integers(len(bks))), int(rng.integers(1, 3))) for _ in range(120)]
+        ts = [ctx.submit(ids[i], k) for i, k in reqs]       # in the mix
+        for t, (i, k) in zip(ts, reqs):
+            assert ctx.wait(t) == 0
+            assert same_bits(ctx.result(t, bks[i].n, k), ref[(i, k)]), (threads, i, k)
+            gpu.call("bs_result_release", ctx.h, t)
+        st = ctx.stats()
+        assert st.requests_failed == 0 and st.launches < st.requests_done
+    finally:
+        ctx.destroy()
+
+
 @pytest.mark.parametrize("use_graph", [1, 0])
 def test_capped_grid_loops_over_tiles(gpu, use_graph):
     """grid_blocks = 7: each section's blocks loop over the batch's tiles;
