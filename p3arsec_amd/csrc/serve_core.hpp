@@ -16,6 +16,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <deque>
 #include <memory>
 #include <unordered_map>
@@ -23,6 +24,31 @@
 
 namespace bs {
 namespace serve {
+
+// A FIFO ticket lock for the serving loop's state.  The loop's threads
+// (clients submitting and releasing, the dispatcher admitting and retiring)
+// each hold it for microseconds at a time, thousands of times a second; with
+// a sleeping mutex every contended hand-over is a futex wake-up of 5-30 us,
+// and at 1.9M requests/s those wake-ups, not the work under the lock, set the
+// small requests' latency.  A waiter spins with the CPU's pause hint for a
+// while, then yields; hand-overs go in arrival order, so no thread starves.
+// Waits that may be long (a client waiting for a ticket, an idle dispatcher)
+// sleep on a condition variable instead, which releases the lock.
+class TicketLock {
+   public:
+    void lock() {
+        const uint32_t me = next_.fetch_add(1, std::memory_order_relaxed);
+        if (serving_.load(std::memory_order_acquire) != me) wait_turn(me);
+    }
+    void unlock() {   // only the holder writes serving_
+        serving_.store(serving_.load(std::memory_order_relaxed) + 1, std::memory_order_release);
+    }
+
+   private:
+    void wait_turn(uint32_t me);
+    alignas(64) std::atomic<uint32_t> next_{0};
+    alignas(64) std::atomic<uint32_t> serving_{0};
+};
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
@@ -179,6 +205,12 @@ class Core {
     void fail_queued(int status, int64_t now);
     int release_result(int64_t ticket);
     Request* request(int64_t ticket);
+    // Starts loading a ticket's record into this core's cache (records are
+    // written by the other threads of the loop in turn).
+    void prefetch(int64_t ticket) const {
+        const int64_t slot = ticket & ((int64_t(1) << kSlotBits) - 1);
+        if (ticket > 0 && slot < nslots_) __builtin_prefetch(&chunks_[slot / kChunk][slot % kChunk], 1);
+    }
 
     // Completion queue: when enabled, every request that finishes (done,
     // failed or cancelled) is appended in the order it finished; reap() pops

@@ -28,7 +28,8 @@
 // request's batch; the executor runs the two lanes on different streams.
 // Each lane admits in FIFO order.
 //
-// Locking: `mu_` guards the Core, `fatal_` and `batch_ns_`.  The dispatcher
+// Locking: `mu_` (a FIFO ticket lock, serve_core.hpp) guards the Core,
+// `fatal_` and `batch_ns_`.  The dispatcher
 // takes it only to admit a batch or retire one; while batches run it queries
 // them with the lock released, so submitting and reaping threads never wait
 // behind the poll loop.  `gen_` counts the changes that can unblock admission
@@ -91,7 +92,7 @@ class Loop {
     void stop() {
         if (!worker_.joinable()) return;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             shutdown_.store(true);
             gen_.fetch_add(1);
         }
@@ -102,14 +103,14 @@ class Loop {
 
     // ---- books ---------------------------------------------------------------
     int register_book(int64_t n, int64_t* id, size_t* off) {
-        std::lock_guard<std::mutex> lk(mu_);
+        std::lock_guard<TicketLock> lk(mu_);
         if (shutdown_.load()) return kErrShutdown;
         return core_.register_book(n, id, off);
     }
     int release_book(int64_t id) {
         int rc;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             rc = core_.release_book(id);
             if (rc == kOk) changed();
         }
@@ -121,7 +122,7 @@ class Loop {
     int submit(int64_t book, int kind, int64_t* ticket) {
         int rc;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             if (shutdown_.load()) return kErrShutdown;
             rc = core_.submit(book, kind, steady_ns(), ticket);
             if (rc == kOk) changed();
@@ -134,7 +135,7 @@ class Loop {
         if (count == 0) return kOk;
         int rc;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             if (shutdown_.load()) return kErrShutdown;
             rc = core_.submit_many(books, kinds, users, count, steady_ns(), tickets);
             if (rc == kOk) changed();
@@ -143,7 +144,7 @@ class Loop {
         return rc;
     }
     int poll(int64_t ticket, int* state) {
-        std::lock_guard<std::mutex> lk(mu_);
+        std::lock_guard<TicketLock> lk(mu_);
         const Request* r = core_.request(ticket);
         if (!r) return kErrNotFound;
         *state = r->state;
@@ -153,7 +154,7 @@ class Loop {
     // forever).  Returns kOk for a done request, else its failure; *msg gets
     // the executor's error text for a failed batch.
     int wait(int64_t ticket, int64_t timeout_us, std::string* msg) {
-        std::unique_lock<std::mutex> lk(mu_);
+        std::unique_lock<TicketLock> lk(mu_);
         auto finished = [&]() {
             const Request* r = core_.request(ticket);
             return !r || (r->state != kQueued && r->state != kRunning);
@@ -176,7 +177,7 @@ class Loop {
         *count = 0;
         const auto deadline =
             std::chrono::steady_clock::now() + std::chrono::microseconds(timeout_us);
-        std::unique_lock<std::mutex> lk(mu_);
+        std::unique_lock<TicketLock> lk(mu_);
         if (!core_.completions_enabled()) return kErrInvalid;
         if (max == 0) return kOk;
         for (;;) {
@@ -207,7 +208,7 @@ class Loop {
     int release(int64_t ticket) {
         int rc;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             rc = core_.release_result(ticket);
             if (rc == kOk) changed();
         }
@@ -218,8 +219,9 @@ class Loop {
     int release_many(const int64_t* tickets, int64_t count) {
         int first = kOk;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             for (int64_t i = 0; i < count; ++i) {
+                if (i + 8 < count) core_.prefetch(tickets[i + 8]);
                 const int rc = core_.release_result(tickets[i]);
                 if (rc != kOk && first == kOk) first = rc;
             }
@@ -231,7 +233,7 @@ class Loop {
     int cancel(int64_t ticket) {
         int rc;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             rc = core_.cancel(ticket, steady_ns());
             if (rc == kOk) {
                 changed();
@@ -245,7 +247,7 @@ class Loop {
         return rc;
     }
     int times(int64_t ticket, int64_t* submit_ns, int64_t* done_ns) {
-        std::lock_guard<std::mutex> lk(mu_);
+        std::lock_guard<TicketLock> lk(mu_);
         const Request* r = core_.request(ticket);
         if (!r) return kErrNotFound;
         if (submit_ns) *submit_ns = r->t_submit;
@@ -254,7 +256,7 @@ class Loop {
     }
     // A done request's size, kind and output offset (for result access).
     int result_info(int64_t ticket, int64_t* n, int* kind, size_t* out_off) {
-        std::lock_guard<std::mutex> lk(mu_);
+        std::lock_guard<TicketLock> lk(mu_);
         const Request* r = core_.request(ticket);
         if (!r) return kErrNotFound;
         if (r->state != kDone) return kErrInvalid;
@@ -264,7 +266,7 @@ class Loop {
         return kOk;
     }
     void stats(Stats* st, size_t* cap, size_t* used, size_t* peak, LoopTimes* t) {
-        std::lock_guard<std::mutex> lk(mu_);
+        std::lock_guard<TicketLock> lk(mu_);
         *st = core_.stats();
         *cap = core_.arena().capacity();
         *used = core_.arena().used();
@@ -296,7 +298,7 @@ class Loop {
                       int64_t t_launch) {
         const int64_t t = steady_ns();
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             if (!ok) fatal_ = err;
             core_.complete(tickets, ok, t);
             if (t_launch >= 0) batch_ns_ += t - t_launch;
@@ -351,7 +353,7 @@ class Loop {
                 for (const Batch& b : inflight) bulk_inflight += b.slot >= Exec::kSmallSlots;
                 const int64_t ta = steady_ns();
                 {
-                    std::lock_guard<std::mutex> lk(mu_);
+                    std::lock_guard<TicketLock> lk(mu_);
                     const int64_t g = gen_.load(std::memory_order_relaxed);
                     const int64_t qo = core_.queued_options(lane);
                     const bool split = lane == kLaneBulk && bulk_inflight < 2 && qo > kSplitOptions;
@@ -387,7 +389,7 @@ class Loop {
                 continue;
             }
             if (shutdown_.load(std::memory_order_acquire)) break;
-            std::unique_lock<std::mutex> lk(mu_);
+            std::unique_lock<TicketLock> lk(mu_);
             cv_work_.wait(lk, [&] {
                 if (shutdown_.load(std::memory_order_relaxed)) return true;
                 const int64_t g = gen_.load(std::memory_order_relaxed);
@@ -398,7 +400,7 @@ class Loop {
         }
         ex_->drain();
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<TicketLock> lk(mu_);
             core_.fail_queued(kErrShutdown, steady_ns());
             changed();
             finished_.fetch_add(1, std::memory_order_release);
@@ -408,9 +410,9 @@ class Loop {
 
     Exec* ex_;
     Core core_;
-    std::mutex mu_;
-    std::condition_variable cv_work_;   // dispatcher: new requests / memory / shutdown
-    std::condition_variable cv_done_;   // clients: ticket state changes
+    TicketLock mu_;
+    std::condition_variable_any cv_work_;   // dispatcher: new requests / memory / shutdown
+    std::condition_variable_any cv_done_;   // clients: ticket state changes
     std::string fatal_;
     int64_t batch_ns_ = 0;              // sum of launch -> completion seen, over batches
     std::atomic<int64_t> gen_{0};
