@@ -851,7 +851,15 @@ def main() -> int:
                        "roofline_frac_100": alg_bytes / (ms_one100 * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("price", n)},
+                     "traffic": measured_traffic("price", n),
+                     "kernel": "k_stream<PriceF32<Ref32, 2>> (bs_price_f32)",
+                     "duration_us": ms * 1e3,
+                     "one_stream_launch_us": ms_one * 1e3,
+                     "note": "duration_us is the time per launch with launches alternating over "
+                             "two streams (each overlaps its neighbours, so a trace's start-to-end "
+                             "average per dispatch spans about two passes and the trace's window "
+                             "per dispatch is the comparable figure); one_stream_launch_us is the "
+                             "launch's own average duration with every launch on one stream"},
     }
     prof = profile_record("price", n)
     if "valu_issue_us" in prof:
