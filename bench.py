@@ -82,7 +82,8 @@ LAUNCH_CPU_US = {}
 # Streams the passes of each map alternate over, the faster schedule measured
 # on MI355X (10M options, 300 passes): price 37.25 / 37.52 us per pass on two
 # streams vs 37.98 / 39.41 on one (two boxes); Greeks 67.22 vs 67.98; FP64
-# 85 vs 89.
+# 85 vs 89.  Over 20 passes the two schedules of the price map are within
+# about 1 us of each other, either way round depending on the box.
 STREAMS = {"price": 2, "fast": 2, "greeks": 2, "fp64": 2}
 
 
@@ -151,37 +152,43 @@ def timed_passes(launch, nbooks: int, steps: int, warmup: int, streams, tag: str
     """Runs warmup + steps passes, pass i on book i % nbooks and stream
     i % len(streams) (launch(i, stream)); returns ms per pass of the timed
     passes.  After a host synchronize, the warmup passes are enqueued, every
-    stream is joined on the device, then the start event, the timed passes,
-    a join of every stream and the end event; a host synchronize follows.  The
-    start event fires when the last warmup pass ends, with the timed launches
-    already queued behind it (the warmup passes cover the host's launch time),
-    so the region holds exactly the timed passes and no host launch latency.
+    stream is joined on the device into the stream of the first timed pass,
+    then the start event on that stream, the timed passes (every other stream
+    waits for the start event first), a join of every stream into the stream
+    of the last timed pass and the end event on it; a host synchronize
+    follows.  The start event fires when the last warmup pass ends, with the
+    timed launches already queued behind it (the warmup passes cover the
+    host's launch time), so the region holds exactly the timed passes and no
+    host launch latency; and the first timed pass follows the start event on
+    its own stream, so no cross-stream wait sits at either end of the region.
     Passes on different streams touch different books and outputs (nbooks is
     a multiple of the stream count), so with two streams one pass's last waves
     overlap the next pass's first ones.  Also records the host time per launch
     call (LAUNCH_CPU_US[tag])."""
     assert nbooks % len(streams) == 0
     ns = len(streams)
+    first, last = streams[warmup % ns], streams[(warmup + steps - 1) % ns]
     capi.call("bs_device_sync")
     for i in range(warmup):
         launch(i % nbooks, streams[i % ns].handle)
     e0, e1 = capi.Event(), capi.Event()
     pre = [capi.Event() for _ in streams[1:]]
     joins = [capi.Event() for _ in streams[1:]]
-    for s, ev in zip(streams[1:], pre):           # every warmup pass ends before e0
+    for s, ev in zip([x for x in streams if x is not first], pre):   # warmups end before e0
         ev.record(s.handle)
-        streams[0].wait(ev)
-    e0.record(streams[0].handle)
-    for s in streams[1:]:
-        s.wait(e0)
+        first.wait(ev)
+    e0.record(first.handle)
+    for s in streams:
+        if s is not first:
+            s.wait(e0)
     t0 = time.perf_counter()
     for i in range(warmup, warmup + steps):
         launch(i % nbooks, streams[i % ns].handle)
     t1 = time.perf_counter()
-    for s, ev in zip(streams[1:], joins):
+    for s, ev in zip([x for x in streams if x is not last], joins):
         ev.record(s.handle)
-        streams[0].wait(ev)
-    e1.record(streams[0].handle)
+        last.wait(ev)
+    e1.record(last.handle)
     e1.sync()
     capi.call("bs_device_sync")
     ms = e0.elapsed_ms(e1) / steps
@@ -807,8 +814,8 @@ def main() -> int:
     spin_up()
     if args.only:
         return only_one(args, L, host, dbooks, outs, streams)
-    s_price = streams[:STREAMS["price"]]
     log("bench: price map")
+    s_price = streams[:STREAMS["price"]]
     prewarm(launch_price, nb, s_price, args.prewarm_ms, "price")
     ms = timed_passes(launch_price, nb, args.steps, args.warmup, s_price, "price")
     # the reference's NUM_RUNS = 100 passes, timed as one region right after
@@ -840,7 +847,7 @@ def main() -> int:
         "config": {"workload": "Black-Scholes price map, 10M-option FP32 book, SoA "
                                "(5 x f32 + u8 in, f32 out), one launch per pass",
                    "options": n, "rotating_books": nb, "bytes_per_option": 25,
-                   "streams": STREAMS["price"], "max_abs_err_vs_oracle_sample": max_err},
+                   "streams": len(s_price), "max_abs_err_vs_oracle_sample": max_err},
         "passes_100": {"ms_per_pass": ms100, "options_per_s": n / (ms100 * 1e-3),
                        "roofline_frac": alg_bytes / (ms100 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "note": "the reference's NUM_RUNS = 100 passes as one timed region, "
@@ -855,6 +862,7 @@ def main() -> int:
                      "kernel": "k_stream<PriceF32<Ref32, 2>> (bs_price_f32)",
                      "duration_us": ms * 1e3,
                      "one_stream_launch_us": ms_one * 1e3,
+                     "streams": len(s_price),
                      "note": "duration_us is the time per launch with launches alternating over "
                              "two streams (each overlaps its neighbours, so a trace's start-to-end "
                              "average per dispatch spans about two passes and the trace's window "
