@@ -10,8 +10,9 @@ shutdown finishes the batches in flight and fails the rest; admission waits
 for released results when the arena is full; the arena returns to the books'
 bytes when every result is released; with lanes on, every batch holds one
 lane's requests in that lane's slots, and small requests finish while a slow
-bulk batch is still running.  The same scenarios run again under
-ThreadSanitizer."""
+bulk batch is still running; the context's ticket lock excludes and serves
+eight threads in turn, one of them sleeping while it holds it.  The same
+scenarios run again under ThreadSanitizer."""
 import os
 import shutil
 import subprocess
@@ -20,7 +21,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NATIVE = os.path.join(HERE, "native")
-SCENARIOS = ["stress", "faults", "shutdown", "books", "memory", "lanes"]
+SCENARIOS = ["stress", "faults", "shutdown", "books", "memory", "lanes", "lock"]
 
 
 def build(target):
