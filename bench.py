@@ -161,19 +161,23 @@ def timed_passes(launch, nbooks: int, steps: int, warmup: int, streams, tag: str
     host's launch time), so the region holds exactly the timed passes and no
     host launch latency; and the first timed pass follows the start event on
     its own stream, so no cross-stream wait sits at either end of the region.
-    Passes on different streams touch different books and outputs (nbooks is
-    a multiple of the stream count), so with two streams one pass's last waves
-    overlap the next pass's first ones.  Also records the host time per launch
-    call (LAUNCH_CPU_US[tag])."""
+    The events are timing-only (bs_event_create_timing); the host synchronize
+    after the region makes the results visible.  Passes on different streams
+    touch different books and outputs (nbooks is a multiple of the stream
+    count), so with two streams one pass's last waves overlap the next pass's
+    first ones.  Also records the host time per launch call
+    (LAUNCH_CPU_US[tag])."""
     assert nbooks % len(streams) == 0
     ns = len(streams)
     first, last = streams[warmup % ns], streams[(warmup + steps - 1) % ns]
     capi.call("bs_device_sync")
     for i in range(warmup):
         launch(i % nbooks, streams[i % ns].handle)
-    e0, e1 = capi.Event(), capi.Event()
-    pre = [capi.Event() for _ in streams[1:]]
-    joins = [capi.Event() for _ in streams[1:]]
+    # timing-only events: no system-scope fence (cache writeback and
+    # invalidation) at the markers inside or at the ends of the region
+    e0, e1 = capi.Event(timing=True), capi.Event(timing=True)
+    pre = [capi.Event(timing=True) for _ in streams[1:]]
+    joins = [capi.Event(timing=True) for _ in streams[1:]]
     for s, ev in zip([x for x in streams if x is not first], pre):   # warmups end before e0
         ev.record(s.handle)
         first.wait(ev)

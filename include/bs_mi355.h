@@ -248,6 +248,12 @@ int bs_stream_sync(bs_stream_t stream);
 int bs_device_sync(void);
 typedef void *bs_event_t;
 int bs_event_create(bs_event_t *ev);
+/* An event for timing only: its record performs no system-scope fence (no
+ * cache writeback and invalidation at the marker, so the time between two
+ * such events holds the work between them and not the fences).  bs_event_sync on it waits for the GPU to pass the marker but does
+ * not make the work's results visible to the host: synchronize the stream or
+ * device before reading them. */
+int bs_event_create_timing(bs_event_t *ev);
 int bs_event_destroy(bs_event_t ev);
 int bs_event_record(bs_event_t ev, bs_stream_t stream);
 int bs_event_sync(bs_event_t ev);

@@ -114,6 +114,7 @@ _SIGS = {
     "bs_stream_wait_event": ([_vp, _vp], _i32),
     "bs_device_sync": ([], _i32),
     "bs_event_create": ([ctypes.POINTER(_vp)], _i32),
+    "bs_event_create_timing": ([ctypes.POINTER(_vp)], _i32),
     "bs_event_destroy": ([_vp], _i32),
     "bs_event_record": ([_vp, _vp], _i32),
     "bs_event_sync": ([_vp], _i32),
@@ -260,9 +261,11 @@ class Stream:
 
 
 class Event:
-    def __init__(self):
+    """A HIP event; timing=True makes a timing-only event (bs_event_create_timing)."""
+
+    def __init__(self, timing: bool = False):
         p = _vp()
-        call("bs_event_create", ctypes.byref(p))
+        call("bs_event_create_timing" if timing else "bs_event_create", ctypes.byref(p))
         self.handle = p.value
 
     def record(self, stream=None) -> None:

@@ -627,6 +627,13 @@ int bs_event_create(bs_event_t* ev) {
     *ev = e;
     return done(rc, "hipEventCreate");
 }
+int bs_event_create_timing(bs_event_t* ev) {
+    if (!ev) return set_error(BS_ERR_INVALID, "ev is NULL");
+    hipEvent_t e = nullptr;
+    hipError_t rc = hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    *ev = e;
+    return done(rc, "hipEventCreateWithFlags");
+}
 int bs_event_destroy(bs_event_t ev) {
     return done(hipEventDestroy(static_cast<hipEvent_t>(ev)), "hipEventDestroy");
 }

@@ -79,6 +79,7 @@ def test_argument_errors_before_device_work():
     assert L.bs_price_passes_f32(*six, 1, None, 0, 100, None) == 0      # empty book
     assert L.bs_price_passes_f32(*six, 1, None, 16, 100, None) == -1    # NULL arrays
     assert L.bs_price_passes_f32(p, p, p, p, p, p, 1, p, 16, 2, None) == -1   # host memory
+    assert L.bs_event_create(None) == -1 and L.bs_event_create_timing(None) == -1
 
 
 def test_scalar_and_thread_compat_arguments():
