@@ -320,7 +320,7 @@ struct Ref64 {
     __device__ static __forceinline__ double sqrt(double x) { return ::sqrt(x); }
 
     template <bool kGrad>
-    __device__ static __forceinline__ double cndf(double x, double* grad) {
+    __device__ static __forceinline__ double cndf(double x, double* /*grad*/) {
 #pragma clang fp contract(off)
         static_assert(!kGrad, "Ref64 prices only");
         const bool neg = x < 0.0;

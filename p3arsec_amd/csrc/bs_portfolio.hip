@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kPfChunkThreads) void k_pf_chunks(const double* __r
     }
     __syncthreads();
     for (int half = kPfChunkThreads / 2; half >= 1; half >>= 1) {
-        if (threadIdx.x < half)
+        if ((int)threadIdx.x < half)
 #pragma unroll
             for (int q = 0; q < NQ; ++q) lds[q][threadIdx.x] += lds[q][threadIdx.x + half];
         __syncthreads();
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kPfFinalThreads) void k_pf_final(const double* __re
     for (int q = 0; q < NQ; ++q) lds[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int half = kPfFinalThreads / 2; half >= 1; half >>= 1) {
-        if (threadIdx.x < half)
+        if ((int)threadIdx.x < half)
 #pragma unroll
             for (int q = 0; q < NQ; ++q) lds[q][threadIdx.x] += lds[q][threadIdx.x + half];
         __syncthreads();

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(kSwThreads) void k_sw_final(const double* __restric
     lds[1][threadIdx.x] = b;
     __syncthreads();
     for (int half = kSwThreads / 2; half >= 1; half >>= 1) {
-        if (threadIdx.x < half) {
+        if ((int)threadIdx.x < half) {
             lds[0][threadIdx.x] += lds[0][threadIdx.x + half];
             lds[1][threadIdx.x] += lds[1][threadIdx.x + half];
         }
