@@ -2,7 +2,8 @@
 pricing functions concurrently (BS:315-348, :607-658), so the C ABI must be
 reentrant.  Eight host threads (ctypes releases the GIL during each call)
 price their own books on their own streams, and call the scalar forms, in a
-loop; every result must equal the single-threaded one bit for bit."""
+loop; every result must equal the single-threaded one bit for bit.  Also the
+timing-only events that bracket passes on two streams."""
 import ctypes
 import threading
 
@@ -64,3 +65,48 @@ def test_eight_threads_on_their_own_streams(gpu):
     assert not errors, errors
     for db in dbs:
         db.free()
+
+
+def test_timing_events_bracket_passes_on_two_streams(gpu):
+    """bs_event_create_timing events (no system-scope fence), as bench.py
+    brackets its timed regions: a region of 8 passes alternating over two
+    streams, joined at both ends, measures a positive time no shorter than
+    one pass timed alone; the outputs read after a device synchronize are the
+    map's own bits."""
+    L = gpu.load()
+    n = 2_000_000
+    book = books.make_book(n, books.SEED_BASE + 150)
+    dbs = [gpu.DeviceBook(book) for _ in range(2)]
+    outs = [gpu.alloc_out(n) for _ in range(2)]
+    ref_out = gpu.price(dbs[0])
+    want = ref_out.download(np.float32, n)
+    st = [gpu.Stream(), gpu.Stream()]
+    one0, one1 = gpu.Event(timing=True), gpu.Event(timing=True)
+    one0.record(st[0].handle)
+    assert L.bs_price_f32(*dbs[0].ptrs(), outs[0].ptr, n, st[0].handle) == 0
+    one1.record(st[0].handle)
+    one1.sync()
+    single = one0.elapsed_ms(one1)
+    e0, e1, pre, join = (gpu.Event(timing=True) for _ in range(4))
+    pre.record(st[1].handle)
+    st[0].wait(pre)
+    e0.record(st[0].handle)
+    st[1].wait(e0)
+    for i in range(8):
+        assert L.bs_price_f32(*dbs[i % 2].ptrs(), outs[i % 2].ptr, n, st[i % 2].handle) == 0
+    join.record(st[0].handle)
+    st[1].wait(join)
+    e1.record(st[1].handle)
+    e1.sync()
+    region = e0.elapsed_ms(e1)
+    gpu.call("bs_device_sync")
+    print(f"\none pass {single * 1e3:.1f} us, 8 passes on two streams {region * 1e3:.1f} us")
+    assert 0.0 < single <= region
+    for o in outs:
+        assert np.array_equal(o.download(np.float32, n).view(np.uint32), want.view(np.uint32))
+    for ev in (one0, one1, e0, e1, pre, join):
+        ev.destroy()
+    for x in dbs + outs + [ref_out]:
+        x.free()
+    for s in st:
+        s.destroy()
