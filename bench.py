@@ -701,6 +701,43 @@ def shard_section(L, host, n, streams) -> dict:
             "note": "one book reused every pass: MALL-resident working set"}
 
 
+def repack_section(L, host, n, streams) -> dict:
+    """The reference's AoS -> SoA repack (BS:747-767) on the device over the
+    10M book: 36-byte OptionData records in, five float arrays and the type
+    bytes out (57 B per option), two rotating record buffers (720 MB) on two
+    streams, timed like the map; the output of the last pass is checked
+    against the book bit for bit."""
+    rec = np.dtype([("s", "<f4"), ("strike", "<f4"), ("r", "<f4"), ("divq", "<f4"),
+                    ("v", "<f4"), ("t", "<f4"), ("OptionType", "S1"), ("pad", "V3"),
+                    ("divs", "<f4"), ("DGrefval", "<f4")])
+    aos = np.zeros(n, rec)
+    for f, a in (("s", host.s), ("strike", host.k), ("r", host.r), ("v", host.v),
+                 ("t", host.t)):
+        aos[f] = a
+    aos["OptionType"] = np.where(host.otype != 0, b"P", b"C")
+    daos = [capi.DeviceBuffer.from_array(aos.view(np.uint8)) for _ in range(2)]
+    del aos
+    outs = [[capi.DeviceBuffer(n * 4) for _ in range(5)] + [capi.DeviceBuffer(n)]
+            for _ in range(2)]
+
+    def launch(i, st):
+        capi.check(L.bs_repack_aos_f32(daos[i].ptr, n, *[o.ptr for o in outs[i]], st), "repack")
+    ss = streams[:2]
+    prewarm(launch, 2, ss, ARGS.prewarm_ms / 2, "repack_aos")
+    t = timed_passes(launch, 2, ARGS.steps, ARGS.warmup, ss)
+    last = (ARGS.warmup + ARGS.steps - 1) % 2
+    same = all(np.array_equal(o.download(np.float32, n).view(np.uint32), a.view(np.uint32))
+               for o, a in zip(outs[last][:5], (host.s, host.k, host.r, host.v, host.t)))
+    same = same and np.array_equal(outs[last][5].download(np.uint8, n),
+                                   (host.otype != 0).astype(np.uint8))
+    for b in daos + [o for row in outs for o in row]:
+        b.free()
+    gbs = 57 * n / (t * 1e-3) / 1e9
+    return {"us_per_pass": t * 1e3, "options_per_s": n / (t * 1e-3), "achieved_gbs": gbs,
+            "roofline_frac": gbs / HBM_PEAK_GBS, "bytes_per_option": 57,
+            "output_equals_book": bool(same)}
+
+
 def host_buffers_section(L, host, n) -> dict:
     """The host-buffer boundary (the SkePU2 Map form): upload, one pass,
     download."""
@@ -733,6 +770,11 @@ def only_one(args, L, host, dbooks, outs, streams) -> int:
                           streams[:1])
         print(json.dumps({"kernel": "copy", "bytes": nbytes, "ms_per_step": ms,
                           "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
+    if args.only == "repack":
+        r = repack_section(L, host, n, streams)
+        print(json.dumps({"kernel": "repack", "options": n, **r,
+                          "prewarm_ms": PREWARM_MS.get("repack_aos")}), flush=True)
         return 0
     if args.only == "price":
         fn = lambda i, s: L.bs_price_f32(*dbooks[i].ptrs(), outs[i].ptr, n, s)  # noqa
@@ -781,7 +823,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--no-swaptions", action="store_true")
     ap.add_argument("--only", choices=["price", "fast", "greeks", "fp64", "copy", "portfolio",
-                                       "portfolio_greeks", "swaptions"],
+                                       "portfolio_greeks", "swaptions", "repack"],
                     help="time one kernel only (for profiler runs)")
     ap.add_argument("--one-stream", action="store_true",
                     help="with --only: every pass on one stream")
@@ -890,6 +932,7 @@ def main() -> int:
                          ("portfolio", lambda: portfolio_section(L, dbooks, n, streams, ms)),
                          ("config_4096", lambda: config4096_section(L, streams)),
                          ("shard_1_25m", lambda: shard_section(L, host, n, streams)),
+                         ("repack_aos_10m", lambda: repack_section(L, host, n, streams)),
                          ("host_buffers", lambda: host_buffers_section(L, host, n))):
             try:
                 log(f"bench: {name}")

@@ -237,8 +237,13 @@ def test_host_pointer_form_and_thread_compat(gpu, oracle):
     assert np.all(prices[3 * blk:] == -1.0)
 
 
-def test_repack_aos(gpu):
-    n = 5000
+@pytest.mark.parametrize("n,offset", [(5000, 0), (1, 0), (255, 0), (256, 0), (257, 0),
+                                      (300_001, 0), (5000, 4), (2048, 4)])
+def test_repack_aos(gpu, n, offset):
+    """Every field of every record lands in its array, bit for bit, for books
+    that fill the kernel's 256-record steps, end inside one, or are smaller
+    than one; offset = 4 places the records 4 bytes past a 16-byte boundary
+    (the reference's records are only 4-byte aligned)."""
     book = books.make_book(n, books.SEED_BASE + 25)
     rec = np.dtype([("s", "<f4"), ("strike", "<f4"), ("r", "<f4"), ("divq", "<f4"),
                     ("v", "<f4"), ("t", "<f4"), ("OptionType", "S1"), ("pad", "V3"),
@@ -252,14 +257,18 @@ def test_repack_aos(gpu):
     pick = np.arange(n) % 4
     aos["OptionType"] = chars[pick]
     aos["divq"] = 7.0
-    daos = gpu.DeviceBuffer.from_array(aos.view(np.uint8))
+    raw = np.zeros(offset + aos.nbytes, np.uint8)
+    raw[offset:] = aos.view(np.uint8)
+    daos = gpu.DeviceBuffer.from_array(raw)
     outs = [gpu.DeviceBuffer(n * 4) for _ in range(5)]
     dty = gpu.DeviceBuffer(n)
-    gpu.call("bs_repack_aos_f32", daos.ptr, n, *[o.ptr for o in outs], dty.ptr, None)
+    gpu.call("bs_repack_aos_f32", daos.ptr + offset, n, *[o.ptr for o in outs], dty.ptr, None)
     gpu.call("bs_stream_sync", None)
     for o, a in zip(outs, (book.s, book.k, book.r, book.v, book.t)):
-        assert np.array_equal(o.download(np.float32, n), a)
+        assert np.array_equal(o.download(np.float32, n).view(np.uint32), a.view(np.uint32))
     assert np.array_equal(dty.download(np.uint8, n), (pick == 0).astype(np.uint8))
+    for b in [daos, dty] + outs:
+        b.free()
 
 
 def test_invalid_pointers_rejected_on_device(gpu):
