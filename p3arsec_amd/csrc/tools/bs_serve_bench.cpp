@@ -40,9 +40,14 @@ int64_t now_ns() {
         .count();
 }
 
+// Ends the process at once: a failed call can come from either client thread
+// while the other and the context's dispatcher are still running, and exit()
+// would run static destructors (the GPU runtime's among them) under them.
 void die(const char* what, int rc) {
     fprintf(stderr, "%s failed: %d (%s) %s\n", what, rc, bs_status_string(rc), bs_last_error());
-    exit(1);
+    fflush(stdout);
+    fflush(stderr);
+    _Exit(1);
 }
 #define CHECK(call)                      \
     do {                                 \
