@@ -343,7 +343,10 @@ int bs_submit(bs_ctx_t ctx, bs_book_t book, int kind, bs_ticket_t *ticket);
 /* Submits count requests under one lock, all or none: an unknown book, a bad
  * kind or outputs that can never fit fail the call before any is queued.
  * Requests of one call join the queue in order.  users (may be NULL) gives
- * each request a tag that its bs_completion carries back. */
+ * each request a tag that its bs_completion carries back.  A context holds
+ * at most 2^24 requests between submit and release (queued, running or
+ * finished and not yet released); past that, bs_submit and bs_submit_many
+ * return BS_ERR_NOMEM. */
 int bs_submit_many(bs_ctx_t ctx, const bs_book_t *books, const int *kinds,
                    const int64_t *users, int64_t count, bs_ticket_t *tickets);
 /* With opts.completion_queue set: pops up to max finished requests in the
