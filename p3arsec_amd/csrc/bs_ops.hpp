@@ -16,7 +16,9 @@
 // two streams, with the agent-scope stores below: price (Ref32) 64 threads,
 // no LDS reserved 37.98 / 37.25 us (6 KB per wave: 38.64 / 37.90; 7 KB:
 // 40.1 / 39.3); Fast32 64 threads, 6 KB 37.56 / 36.95 us (none: 39.06 /
-// 37.80); Greeks 64 threads, 11 KB per wave 67.98 / 67.22 us (none: 74.16 /
+// 37.80); Greeks 64 threads, four options per work-item, 16 KB per wave
+// 65.9 us on two streams (14 KB: 65.7-65.9; 18 KB: 67.1; none: 77.1), two
+// options per work-item, 11 KB per wave 67.98 / 67.22 us (none: 74.16 /
 // 73.79; 9 KB: 68.18 / 68.52; 12 KB: 70.82 / 69.59); int32-type price 64
 // threads 44.9 us, FP64 64 threads pipelined 85 us.  Small blocks help:
 // one-wave workgroups retire and refill independently, so the tail of the
@@ -211,7 +213,8 @@ template <int VW>
 struct GreeksF32 {
     static constexpr int kVW = VW;
     static constexpr int kElemBytes = 4;   // widest array element
-    static constexpr int kLdsPerWave = 11264;  // 14 waves per CU
+    // VW = 4 (16-byte stores, the launcher's choice): 10 waves per CU; VW = 2: 14
+    static constexpr int kLdsPerWave = VW == 4 ? 16384 : 11264;
     static constexpr int kWaves = VW == 4 ? 5 : 8;
     static constexpr int kThreadsPref = 64;
     static constexpr int64_t kBlocks = 0;
